@@ -1,0 +1,225 @@
+"""Benchmark of the HMC fermion hot path on one MI355X.
+
+Workload (default): L=36, Ltau=360 (466,560 sites), phi uniform in [-pi, pi),
+J=1, K=1, dtau=0.1, t=1, N_leapfrog=5 (6 PCG solves per trajectory), delta_t=0.06,
+the reference's manual tau-band preconditioner, PCG with the reference's
+graph-mode semantics (fixed max_iter per solve, cg_rtol; converged chains
+freeze).  A "step" is one HMC trajectory of a batch of --bs chains, the whole
+trajectory replayed from one hipGraph (mixed precision: complex64 PCG with
+FP64 dots and FP64 reliable updates every 50 iterations).
+
+Reported next to it:
+  * the same-GPU FP64 plain path: complex128 PCG, one chain per trajectory,
+    also replayed from a hipGraph (s per chain-trajectory);
+  * relative error of the mixed solution / force / Delta H against FP64 on the
+    same field and pseudofermion;
+  * M^dag M (mdagm_kernel, complex64) GB/s from HIP events around back-to-back
+    launches on the launch stream, for --roof-nrhs right-hand sides sharing phi
+    (working set far above the 256 MB Infinity Cache), vs the 8 TB/s HBM peak;
+  * cpu_baseline: the FP64 OpenMP oracle (oracle/qf_oracle.c, a C port of the
+    reference's PCG/force path) timed on this host for a bounded number of PCG
+    iterations and scaled to one trajectory.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [...]; prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--L", type=int, default=36)
+    ap.add_argument("--Ltau", type=int, default=360)
+    ap.add_argument("--bs", type=int, default=4)
+    ap.add_argument("--max-iter", type=int, default=400)
+    ap.add_argument("--rtol", type=float, default=1e-9)
+    ap.add_argument("--ru-period", type=int, default=50)
+    ap.add_argument("--n-leap", type=int, default=5)
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--roof-nrhs", type=int, default=64)
+    ap.add_argument("--roof-reps", type=int, default=20)
+    ap.add_argument("--cpu-iters", type=int, default=0, help="oracle PCG iterations for cpu_baseline (0 = auto, ~15 s)")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=2024)
+    return ap.parse_args()
+
+
+def mdagm_roofline(torch, K, L, Lt, nrhs, reps, dev):
+    """GB/s of the complex64 M^dag M kernel: algorithmic bytes nrhs*(8N read + 8N write) + 8N phi per launch."""
+    N = L * L * Lt
+    g = torch.Generator(device="cpu").manual_seed(7)
+    phi = ((torch.rand(1, 2 * N, generator=g, dtype=torch.float64) * 2 - 1) * math.pi).to(dev, torch.float32)
+    x = torch.randn(nrhs, N, dtype=torch.complex64, device=dev)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        K.apply("MdagM", phi, x, L, Lt, out=y)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        K.apply("MdagM", phi, x, L, Lt, out=y)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    algo = nrhs * 16 * N + 8 * N
+    return {"kernel": "mdagm_kernel<float> (M^dag M, complex64, phi shared)", "nrhs": nrhs, "ms_per_launch": ms,
+            "bytes_per_launch": algo, "GBps": algo / (ms * 1e-3) / 1e9}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (see DESIGN.md, sharding)"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from qed_fermion_amd import kernels as K
+    from qed_fermion_amd.trajectory import TrajectoryEngine
+
+    dev = torch.device("cuda:0")
+    L, Lt, bs = args.L, args.Ltau, args.bs
+    N = L * L * Lt
+    rng = np.random.default_rng(args.seed)
+    phi0 = torch.from_numpy(rng.uniform(-np.pi, np.pi, (bs, 2 * N))).to(dev)
+    common = dict(dtau=0.1, J=1.0, K=1.0, compact=False, m=1.0, delta_t=0.06, n_leap=args.n_leap,
+                  max_iter=args.max_iter, rtol=args.rtol, device=dev)
+
+    # ---------------------------------------------------------- mixed, batched, graph
+    log(f"[bench] mixed engine L={L} Ltau={Lt} bs={bs} max_iter={args.max_iter}")
+    torch.manual_seed(args.seed)
+    eng = TrajectoryEngine(L, Lt, bs, precision="mixed", ru_period=args.ru_period, **common)
+    eng.set_phi(phi0)
+    t0 = time.time()
+    eng.capture(n_warmup=1)
+    torch.cuda.synchronize()
+    log(f"[bench] captured mixed trajectory graph in {time.time() - t0:.1f}s")
+    for _ in range(args.warmup):
+        eng.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.replay()
+    torch.cuda.synchronize()
+    t_mixed = (time.perf_counter() - t0) / args.steps
+    acc = eng.accept.float().mean().item()
+    mixed_err = eng.err.max().item()
+    mixed_iters = eng.iters.mean().item()
+    log(f"[bench] mixed: {t_mixed:.3f} s per batched trajectory ({t_mixed / bs:.3f} s per chain), "
+        f"max solve err {mixed_err:.3e}, mean iters {mixed_iters:.1f}")
+
+    # ---------------------------------------------------------- accuracy vs FP64 on one state
+    phi_s = eng.x[:1].clone()
+    psi = K.apply("Mdag", phi_s, (torch.randn(1, N, dtype=torch.complex128, device=dev) / math.sqrt(2)), L, Lt)
+    ptab64 = K.precon_table(Lt, dtype=torch.float64, device=dev)
+    x64, it64, e64 = K.pcg_solve(phi_s, psi, L, Lt, 0.1, ptab=ptab64, rtol=args.rtol, max_iter=args.max_iter)
+    xm, itm, em = K.pcg_mixed_solve(phi_s, psi, L, Lt, 0.1, ptab=ptab64, rtol=args.rtol, max_iter=args.max_iter,
+                                    ru_period=args.ru_period)
+    F64 = K.fermion_force(phi_s, x64, L, Lt)
+    Fm = K.fermion_force(phi_s, xm, L, Lt)
+    Sf64 = torch.vdot(psi.view(-1), x64.view(-1)).real.item()
+    Sfm = torch.vdot(psi.view(-1), xm.view(-1)).real.item()
+    acc_report = {
+        "solution_rel_err_vs_fp64": (torch.linalg.norm(xm - x64) / torch.linalg.norm(x64)).item(),
+        "force_rel_err_vs_fp64": (torch.linalg.norm(Fm - F64) / torch.linalg.norm(F64)).item(),
+        "Sf_abs_err_vs_fp64": abs(Sfm - Sf64),
+        "residual_fp64": e64.item(), "residual_mixed_true": em.item(),
+        "iters_fp64": it64.item(), "iters_mixed": itm.item(),
+    }
+    log(f"[bench] accuracy: {acc_report}")
+
+    # ---------------------------------------------------------- FP64 plain (one chain per trajectory)
+    log("[bench] fp64 plain engine (bs=1)")
+    torch.manual_seed(args.seed)
+    plain = TrajectoryEngine(L, Lt, 1, precision="fp64", **common)
+    plain.set_phi(phi0[:1])
+    plain.capture(n_warmup=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.plain_steps):
+        plain.replay()
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t0) / args.plain_steps
+    log(f"[bench] fp64 plain: {t_plain:.3f} s per chain-trajectory")
+    del plain
+
+    # ---------------------------------------------------------- M^dag M roofline
+    roof = mdagm_roofline(torch, K, L, Lt, args.roof_nrhs, args.roof_reps, dev)
+    roof_b = mdagm_roofline(torch, K, L, Lt, bs, args.roof_reps * 4, dev)
+    log(f"[bench] M^dag M nrhs={args.roof_nrhs}: {roof['GBps']:.0f} GB/s; nrhs={bs}: {roof_b['GBps']:.0f} GB/s")
+
+    # ---------------------------------------------------------- CPU baseline (oracle)
+    cpu = None
+    if not args.skip_cpu:
+        from oracle import oracle as O
+        phi_np = phi0[0].cpu().numpy().reshape(Lt, L, L, 2)
+        R = (rng.standard_normal(N) + 1j * rng.standard_normal(N)) / 2.0
+        b = O.apply_Mdag(phi_np, R, L, Lt)
+        tab = O.precon_table(Lt)
+        t0 = time.perf_counter()
+        O.pcg(phi_np, b, L, Lt, table=tab, rtol=0.0, max_iter=5, early_exit=False)
+        per_it = (time.perf_counter() - t0) / 5
+        n_it = args.cpu_iters or max(5, min(2000, int(15.0 / max(per_it, 1e-6))))
+        t0 = time.perf_counter()
+        O.pcg(phi_np, b, L, Lt, table=tab, rtol=0.0, max_iter=n_it, early_exit=False)
+        per_it = (time.perf_counter() - t0) / n_it
+        solves = args.n_leap + 1
+        cpu_traj = per_it * args.max_iter * solves
+        cpu = {"value": cpu_traj, "unit": "s/trajectory", "cores": O.num_threads(), "kind": "port",
+               "sample": f"{n_it} FP64 PCG iterations (oracle/qf_oracle.c, OpenMP) at L={L} Ltau={Lt}, "
+                         f"{per_it * 1e3:.1f} ms/iteration, scaled to {solves} solves x {args.max_iter} iterations "
+                         f"(force and boson steps not included)"}
+        log(f"[bench] cpu baseline: {per_it * 1e3:.1f} ms/iter -> {cpu_traj:.1f} s/trajectory")
+
+    value = t_mixed / bs
+    out = {
+        "metric": "HMC trajectory s + M†M GB/s at L=36,Nτ=360: mixed/batched vs FP64 plain",
+        "value": value,
+        "unit": "s/trajectory",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_mixed * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64+f64-reliable",
+        "data": "synthetic: phi uniform in [-pi,pi) from a seed, p ~ N(0,1), R ~ randn(complex)/sqrt(2)",
+        "config": {"workload": f"HMC trajectory, L={L}, Ltau={Lt}, {bs} chains batched, N_leapfrog={args.n_leap}, "
+                               f"PCG max_iter={args.max_iter} rtol={args.rtol}, precon=manual tau band",
+                   "L": L, "Ltau": Lt, "chains": bs, "max_iter": args.max_iter, "rtol": args.rtol,
+                   "ru_period": args.ru_period, "parallelism": "single GPU, chains batched"},
+        "fp64_plain_s_per_trajectory": t_plain,
+        "speedup_vs_fp64_plain": t_plain / value,
+        "mixed_max_residual": mixed_err,
+        "acceptance": acc,
+        "accuracy_vs_fp64": acc_report,
+        "mdagm_batch_GBps": roof_b["GBps"],
+        "roofline": {"bound": "hbm", "achieved": roof["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": roof["GBps"] / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": roof["kernel"], "nrhs": roof["nrhs"], "bytes_per_launch": roof["bytes_per_launch"],
+                     "ms_per_launch": roof["ms_per_launch"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

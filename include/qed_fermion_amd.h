@@ -1,0 +1,136 @@
+/*
+ * qed_fermion_amd.h -- C ABI of libqf_hip.so, the MI355X (gfx950) HMC fermion
+ * hot path of qed_fermion.
+ *
+ * Conventions
+ *  - All array arguments are DEVICE pointers owned by the caller (the Python
+ *    layer passes torch tensors' data_ptr()); the library allocates nothing and
+ *    never synchronises the host, so every call can be captured into a hipGraph.
+ *  - stream is a hipStream_t passed as void* (NULL = default stream).
+ *  - prec: QF_PREC_F32 = complex64 vectors + float32 phi (the reference's
+ *    dtypes, hmc_sampler_batch.py:118-123); QF_PREC_F64 = complex128 + float64.
+ *  - Layouts are the reference's permuted views (hmc_sampler_batch.py:2166):
+ *      phi  [chain][Ltau][L][L][2]  (tau, y, x, mu), chain stride phi_cs reals
+ *           (0 = one phi shared by all right-hand sides)
+ *      vec  [rhs][Ltau][L][L]       complex, rhs stride Ltau*L*L
+ *      Ft   same layout as phi
+ *  - Return value: QF_OK or an error code; qf_last_error() has the message
+ *    (thread-local).  No exceptions cross the ABI.
+ *
+ * File:line citations are relative to the reference repository
+ * (KexinFeng/qed_fermion).
+ */
+#ifndef QED_FERMION_AMD_H
+#define QED_FERMION_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QF_ABI_VERSION 1
+
+enum {
+    QF_OK = 0,
+    QF_ERR_INVALID = 1,     /* bad argument, size or null pointer */
+    QF_ERR_ODD_L = 2,       /* L odd: checkerboard groups (hmc_sampler_batch.py:937-969) need even L */
+    QF_ERR_LTAU = 3,        /* manual preconditioner needs Ltau >= 26 (precon_manual.py:50-167) */
+    QF_ERR_HIP = 4,         /* HIP runtime / launch failure */
+    QF_ERR_UNSUPPORTED = 5, /* lattice shape with no valid work-group tiling */
+    QF_ERR_PRECON = 6       /* CSR preconditioner is not P_tau (x) I_Vs with a 13-band tau stencil */
+};
+
+enum { QF_PREC_F32 = 0, QF_PREC_F64 = 1 };
+
+enum { QF_OP_M = 1, QF_OP_MDAG = 2, QF_OP_B = 3, QF_OP_MDAGM = 4, QF_OP_BPART0 = 16 };
+/* QF_OP_BPART0 + m (m = 0..7): y_tau = Q_m x_tau, the first m factors of the
+ * B_tau sequence D4, D3, D2, D1, D2, D3, D4 -- the intermediates that
+ * _C.b_vec_per_tau(..., interm=True) returns (hmc_sampler_batch.py:2183-2189). */
+
+int qf_version(void);
+const char* qf_last_error(void);
+
+/* Work-group tiling: patch height by in {2,4,6} (0 = auto), tau slices per
+ * operator work-group t (0 = auto), tau slices per update work-group tb
+ * (0 = auto).  Set by the autotuner (qed_fermion_amd/autotune.py); no
+ * counterpart in the reference, whose kernels take BLOCK_SIZE=(4,8)
+ * (hmc_sampler_batch.py:111). */
+int qf_set_tuning(int prec, int by, int t, int tb);
+int qf_get_tiling(int prec, int nrhs, int L, int Ltau, int* out4);
+
+/* y = op(phi) x for nrhs right-hand sides.  op = QF_OP_M: get_M_sparse's M
+ * (hmc_sampler_batch.py:1884-1906); QF_OP_MDAG: M^dag (_C.mh_vec, :2406);
+ * QF_OP_MDAGM: M^dag M (_C.mhm_vec, :772, :799); QF_OP_B: y_tau = B_tau x_tau
+ * per slice (_C.b_vec_per_tau with interm=False, :2181). */
+int qf_apply(int op, int prec, const void* phi, long long phi_cs, const void* x, void* y, int nrhs, int L, int Ltau, double dtau,
+             double t, void* stream);
+
+/* _C.mhm_vec(boson, x, Lx, dtau, *BLOCK_SIZE) (hmc_sampler_batch.py:772, 799): batched y = M^dag M x, t = 1. */
+int qf_mhm_vec(int prec, const void* phi, const void* x, void* y, int bs, int L, int Ltau, double dtau, void* stream);
+
+/* _C.mh_vec(boson, R, Lx, dtau, *BLOCK_SIZE) (hmc_sampler_batch.py:2406; stochastic_estimator.py:388, 430): y = M^dag x. */
+int qf_mh_vec(int prec, const void* phi, const void* x, void* y, int bs, int L, int Ltau, double dtau, void* stream);
+
+/* _C.precon_vec(r, precon_csr, Lx) (hmc_sampler_batch.py:773, 838): z = P r with P given as its
+ * cyclic tau-band table ptab[Ltau][13] (float for F32, double for F64) instead of a CSR. */
+int qf_precon_vec(int prec, const void* ptab, const void* r, void* z, int nrhs, int Vs, int Ltau, void* stream);
+
+/* HOST functions.  get_precon_man(Lx, Ltau) (precon_manual.py:15-175) as a [Ltau][13] table:
+ * table[tau*13 + o + 6] multiplies r_{(tau+o) mod Ltau}. */
+int qf_precon_table(int Ltau, double* table);
+/* Verify that a CSR matrix (reset_precon's precon_csr, hmc_sampler_batch.py:547) is
+ * P_tau (x) I_Vs with a 13-point cyclic tau band and extract its table.  val_im may be NULL. */
+int qf_precon_table_from_csr(const long long* crow, const long long* col, const double* val_re, const double* val_im, long long nrows,
+                             int Vs, int Ltau, double* table);
+
+/* Batched PCG for M^dag M x = b, the recurrence of preconditioned_cg_fast_test
+ * (hmc_sampler_batch.py:751-846): x0 = 0, fixed max_iter with the per-chain
+ * active mask (converged chains freeze; their kernels exit early).  ptab NULL
+ * = no preconditioner.  Outputs on the device: err_out[nrhs] (last relative
+ * residual) and iters_out[nrhs] (the reference's iteration counter).
+ * Two kernels per iteration; all scalars stay on the device.  work: at least
+ * qf_pcg_workspace_bytes(prec, 0, nrhs, L, Ltau) bytes of device memory. */
+size_t qf_pcg_workspace_bytes(int prec, int mixed, int nrhs, int L, int Ltau);
+int qf_pcg_solve(int prec, const void* phi, long long phi_cs, const void* b, void* x, int nrhs, int L, int Ltau, double dtau, double t,
+                 const void* ptab, double rtol, int max_iter, void* work, size_t work_bytes, double* err_out, double* iters_out,
+                 void* stream);
+
+/* Mixed-precision PCG (same recurrence and outputs): complex64 vectors and
+ * operator with FP64 dot products, and every ru_period iterations a reliable
+ * update x_hi += x_lo; r = b - (M^dag M)_FP64 x_hi.  b and x are complex128;
+ * phi64 is the float64 master field and phi32 its float32 copy.  err_out is the
+ * FP64 true residual of the returned x.  Replaces the complex64 CG of
+ * preconditioned_cg_fast_test (hmc_sampler_batch.py:751-846). */
+int qf_pcg_mixed_solve(const double* phi64, const float* phi32, long long phi_cs, const void* b, void* x, int nrhs, int L, int Ltau,
+                       double dtau, double t, const float* ptab32, double rtol, int max_iter, int ru_period, void* work,
+                       size_t work_bytes, double* err_out, double* iters_out, void* stream);
+
+/* Fermion force Ft = -dS_f/dphi given xi = (M^dag M)^{-1} psi: the second half of
+ * force_f_fast (hmc_sampler_batch.py:2174-2227, per-tau _C.b_vec_per_tau calls and
+ * df_dot gathers 1969-2006) fused into one launch.  Ft has the phi layout. */
+int qf_fermion_force(int prec, const void* phi, long long phi_cs, const void* xi, void* Ft, int nrhs, int L, int Ltau, double dtau,
+                     double t, void* stream);
+
+/* Boson sector (FP64, phi layout).  force_b_plaq_matfree + force_b_tau_cmp/noncmp
+ * (hmc_sampler_batch.py:1338-1360, 1438-1459). */
+int qf_boson_force(const double* x, double* Fp, double* Ft, int nrhs, int L, int Ltau, double cJ, double cK, int compact, void* stream);
+/* leapfrog_cmp (hmc_sampler_batch.py:2332-2347): nsub boson sub-steps of dt/nsub per chain;
+ * Fp/Ft hold the forces at x on entry and are updated.  tau_mask may be NULL. */
+int qf_leapfrog_boson(double* x, double* p, double* Fp, double* Ft, const double* dt, const double* tau_mask, double m, int nsub, int nrhs,
+                      int L, int Ltau, double cJ, double cK, int compact, void* stream);
+/* p += scale * dt[chain] * F * tau_mask (fermion kicks, hmc_sampler_batch.py:2505, 2530). */
+int qf_momentum_kick(double* p, const double* F, const double* dt, double scale, const double* tau_mask, int nrhs, int L, int Ltau,
+                     void* stream);
+/* H = S_tau + S_plaq + sum p^2/2m + Re(psi^dag xi) per chain in FP64
+ * (hmc_sampler_batch.py:2417-2424, 2611-2618).  p, psi, xi may be NULL (terms
+ * skipped; psi and xi together).  parts[nrhs][4] optional. */
+size_t qf_hamiltonian_workspace_bytes(int nrhs);
+int qf_hamiltonian(int prec, const double* x, const double* p, const void* psi, const void* xi, double* H, double* parts, int nrhs, int L,
+                   int Ltau, double cJ, double cK, int compact, double m, void* work, size_t work_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,108 @@
+"""ctypes binding of libqf_hip.so (the C ABI declared in include/qed_fermion_amd.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (``make -C
+qed_fermion_amd/csrc``).  There is no fallback: if the shared object is
+missing or a call fails, a :class:`QfError` is raised.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+SO_PATH = os.path.join(_HERE, "libqf_hip.so")
+
+QF_OK = 0
+QF_ERR_INVALID = 1
+QF_ERR_ODD_L = 2
+QF_ERR_LTAU = 3
+QF_ERR_HIP = 4
+QF_ERR_UNSUPPORTED = 5
+QF_ERR_PRECON = 6
+
+QF_PREC_F32 = 0
+QF_PREC_F64 = 1
+
+QF_OP_M = 1
+QF_OP_MDAG = 2
+QF_OP_B = 3
+QF_OP_MDAGM = 4
+
+_ERR_NAMES = {1: "QF_ERR_INVALID", 2: "QF_ERR_ODD_L", 3: "QF_ERR_LTAU", 4: "QF_ERR_HIP",
+              5: "QF_ERR_UNSUPPORTED", 6: "QF_ERR_PRECON"}
+
+
+class QfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_ERR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_LL = ctypes.c_longlong
+_D = ctypes.c_double
+_SZ = ctypes.c_size_t
+
+# name -> (restype, argtypes); mirrors include/qed_fermion_amd.h
+SIGNATURES = {
+    "qf_version": (_I, []),
+    "qf_last_error": (ctypes.c_char_p, []),
+    "qf_set_tuning": (_I, [_I, _I, _I, _I]),
+    "qf_get_tiling": (_I, [_I, _I, _I, _I, _P]),
+    "qf_apply": (_I, [_I, _I, _P, _LL, _P, _P, _I, _I, _I, _D, _D, _P]),
+    "qf_mhm_vec": (_I, [_I, _P, _P, _P, _I, _I, _I, _D, _P]),
+    "qf_mh_vec": (_I, [_I, _P, _P, _P, _I, _I, _I, _D, _P]),
+    "qf_precon_vec": (_I, [_I, _P, _P, _P, _I, _I, _I, _P]),
+    "qf_precon_table": (_I, [_I, _P]),
+    "qf_precon_table_from_csr": (_I, [_P, _P, _P, _P, _LL, _I, _I, _P]),
+    "qf_pcg_workspace_bytes": (_SZ, [_I, _I, _I, _I, _I]),
+    "qf_pcg_solve": (_I, [_I, _P, _LL, _P, _P, _I, _I, _I, _D, _D, _P, _D, _I, _P, _SZ, _P, _P, _P]),
+    "qf_pcg_mixed_solve": (_I, [_P, _P, _LL, _P, _P, _I, _I, _I, _D, _D, _P, _D, _I, _I, _P, _SZ, _P, _P, _P]),
+    "qf_fermion_force": (_I, [_I, _P, _LL, _P, _P, _I, _I, _I, _D, _D, _P]),
+    "qf_boson_force": (_I, [_P, _P, _P, _I, _I, _I, _D, _D, _I, _P]),
+    "qf_leapfrog_boson": (_I, [_P, _P, _P, _P, _P, _P, _D, _I, _I, _I, _I, _D, _D, _I, _P]),
+    "qf_momentum_kick": (_I, [_P, _P, _P, _D, _P, _I, _I, _I, _P]),
+    "qf_hamiltonian_workspace_bytes": (_SZ, [_I]),
+    "qf_hamiltonian": (_I, [_I, _P, _P, _P, _P, _P, _P, _I, _I, _I, _D, _D, _I, _D, _P, _SZ, _P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libqf_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(SO_PATH):
+            raise QfError(QF_ERR_HIP, f"{SO_PATH} is missing: run __graft_entry__.build() "
+                                      "(make -C qed_fermion_amd/csrc) first; there is no CPU fallback")
+        h = ctypes.CDLL(SO_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(h, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = h
+    return _lib
+
+
+def check(rc):
+    if rc != QF_OK:
+        msg = lib().qf_last_error()
+        raise QfError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def call(name, *args):
+    return check(getattr(lib(), name)(*args))
+
+
+def stream_handle():
+    """hipStream_t of torch's current stream on the current device."""
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())

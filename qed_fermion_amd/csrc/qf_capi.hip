@@ -1,0 +1,680 @@
+// qf_capi.hip -- C-ABI of the MI355X HMC fermion library (libqf_hip.so).
+//
+// Every entry point takes raw device pointers, sizes and a hipStream_t passed
+// as void*, launches on that stream only, never synchronises the host and
+// never allocates, so any sequence of calls can be captured into a hipGraph by
+// the caller.  Errors are returned as status codes; qf_last_error() gives the
+// message of the calling thread's last failure.  Declarations and the
+// reference interface each one replaces: include/qed_fermion_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/qed_fermion_amd.h"
+#include "qf_boson.h"
+#include "qf_cg.h"
+#include "qf_force.h"
+#include "qf_operator.h"
+
+using namespace qf;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define QF_HIP(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) return fail(QF_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+#define QF_LAUNCHED()                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = hipGetLastError();                                                     \
+        if (e_ != hipSuccess) return fail(QF_ERR_HIP, "kernel launch: %s", hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------- tiling
+struct Tuning {
+    int by = 0, t = 0, tb = 0;   // 0 = automatic
+};
+Tuning g_tune[2];                // per precision
+std::mutex g_tune_mu;
+
+// Work-groups of up to 512 threads get 256 VGPRs per lane; only the 2x2
+// patch may use 1024-thread work-groups (L up to 64).
+inline int launch_bound(int by) { return by == 2 ? 1024 : 512; }
+inline int nthreads(int L, int by) { return (L / 2) * (L / by); }
+inline bool by_ok(int L, int by) { return by > 0 && L % by == 0 && by % 2 == 0 && nthreads(L, by) <= launch_bound(by); }
+
+struct Tiling {
+    int by, T, nA;     // operator kernels
+    int TB, nsx, nB;   // update kernel
+};
+
+int pick_by(int L, int prec) {
+    int want;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        want = g_tune[prec].by;
+    }
+    if (want && by_ok(L, want)) return want;
+    for (int by : {2, 4, 6})
+        if (by_ok(L, by)) return by;
+    return 0;
+}
+
+Tiling tiling(int prec, int nrhs, int L, int Lt) {
+    Tiling t;
+    Tuning tu;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        tu = g_tune[prec];
+    }
+    t.by = pick_by(L, prec);
+    const long long units = (long long)nrhs * Lt;
+    t.T = tu.t ? tu.t : (int)std::max(1LL, std::min(32LL, units / 1024));
+    t.T = std::min(t.T, Lt);
+    t.nA = (Lt + t.T - 1) / t.T;
+    const int Vs = L * L;
+    t.nsx = (Vs + 255) / 256;
+    t.TB = tu.tb ? tu.tb : (int)std::max(2LL, std::min(64LL, units * t.nsx / 1024));
+    t.TB = std::min(t.TB, Lt);
+    t.nB = t.nsx * ((Lt + t.TB - 1) / t.TB);
+    return t;
+}
+
+template <typename R>
+Hop<R> make_hop(double dtau, double t) {
+    Hop<R> h;
+    h.ch1 = (R)std::cosh(t * dtau);
+    h.sh1 = (R)std::sinh(t * dtau);
+    h.ch2 = (R)std::cosh(0.5 * t * dtau);
+    h.sh2 = (R)std::sinh(0.5 * t * dtau);
+    return h;
+}
+
+int check_lattice(int L, int Lt, int nrhs) {
+    if (L < 2 || (L & 1)) return fail(QF_ERR_ODD_L, "L=%d: the checkerboard bond groups need an even L >= 2", L);
+    if (Lt < 2) return fail(QF_ERR_INVALID, "Ltau=%d must be >= 2", Lt);
+    if (nrhs < 1 || nrhs > 65535) return fail(QF_ERR_INVALID, "nrhs=%d out of range [1, 65535]", nrhs);
+    if (pick_by(L, 0) == 0) return fail(QF_ERR_UNSUPPORTED, "L=%d: no patch shape fits a 512/1024-thread work-group", L);
+    return QF_OK;
+}
+
+// Raise a kernel's dynamic-LDS limit above 64 KB once (not a stream operation,
+// so it is also safe while the caller captures a graph).
+template <typename KernelT>
+int set_lds(KernelT kern, size_t bytes) {
+    static std::mutex mu;
+    static std::vector<std::pair<const void*, size_t>> done;
+    if (bytes <= 65536) return QF_OK;
+    const void* key = reinterpret_cast<const void*>(kern);
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto& d : done)
+        if (d.first == key && d.second >= bytes) return QF_OK;
+    hipError_t e = hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return fail(QF_ERR_HIP, "LDS of %zu bytes refused: %s", bytes, hipGetErrorString(e));
+    done.emplace_back(key, bytes);
+    return QF_OK;
+}
+
+inline dim3 blk_of(int L, int by) { return dim3((unsigned)(((nthreads(L, by) + 63) / 64) * 64)); }
+
+// ---------------------------------------------------------------- launchers
+template <typename R, int BY, int MODE>
+int launch_mdm_by(const MdmArgs<R>& a, int nrhs, hipStream_t s) {
+    const size_t lds = lds_bytes<R>(a.L, 1);
+    const dim3 grid((a.Lt + a.T - 1) / a.T, nrhs), blk = blk_of(a.L, BY);
+    if (blk.x <= 512) {
+        auto k = mdagm_kernel<R, BY, MODE, 512>;
+        int rc = set_lds(k, lds);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k, grid, blk, lds, s, a);
+    } else if constexpr (BY == 2) {
+        auto k = mdagm_kernel<R, BY, MODE, 1024>;
+        int rc = set_lds(k, lds);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k, grid, blk, lds, s, a);
+    }
+    QF_LAUNCHED();
+    return QF_OK;
+}
+
+template <typename R, int MODE>
+int launch_mdm(const MdmArgs<R>& a, int by, int nrhs, hipStream_t s) {
+    switch (by) {
+        case 2: return launch_mdm_by<R, 2, MODE>(a, nrhs, s);
+        case 4: return launch_mdm_by<R, 4, MODE>(a, nrhs, s);
+        case 6: return launch_mdm_by<R, 6, MODE>(a, nrhs, s);
+    }
+    return fail(QF_ERR_UNSUPPORTED, "patch height %d", by);
+}
+
+template <typename R, int BY, int OP>
+int launch_op_by(const OpArgs<R>& a, int nrhs, hipStream_t s) {
+    const size_t lds = lds_bytes<R>(a.L, 1);
+    const dim3 grid((a.Lt + a.T - 1) / a.T, nrhs), blk = blk_of(a.L, BY);
+    if (blk.x <= 512) {
+        auto k = op_kernel<R, BY, OP, 512>;
+        int rc = set_lds(k, lds);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k, grid, blk, lds, s, a);
+    } else if constexpr (BY == 2) {
+        auto k = op_kernel<R, BY, OP, 1024>;
+        int rc = set_lds(k, lds);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k, grid, blk, lds, s, a);
+    }
+    QF_LAUNCHED();
+    return QF_OK;
+}
+
+template <typename R, int OP>
+int launch_op(const OpArgs<R>& a, int by, int nrhs, hipStream_t s) {
+    switch (by) {
+        case 2: return launch_op_by<R, 2, OP>(a, nrhs, s);
+        case 4: return launch_op_by<R, 4, OP>(a, nrhs, s);
+        case 6: return launch_op_by<R, 6, OP>(a, nrhs, s);
+    }
+    return fail(QF_ERR_UNSUPPORTED, "patch height %d", by);
+}
+
+template <typename R, int BY>
+int launch_force_by(const ForceArgs<R>& a, int nrhs, hipStream_t s) {
+    const size_t lds = lds_bytes<R>(a.L, 2);
+    const dim3 grid((a.Lt + a.T - 1) / a.T, nrhs), blk = blk_of(a.L, BY);
+    if (blk.x <= 512) {
+        auto k = force_kernel<R, BY, 512>;
+        int rc = set_lds(k, lds);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k, grid, blk, lds, s, a);
+    } else if constexpr (BY == 2) {
+        auto k = force_kernel<R, BY, 1024>;
+        int rc = set_lds(k, lds);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k, grid, blk, lds, s, a);
+    }
+    QF_LAUNCHED();
+    return QF_OK;
+}
+
+template <typename R, int MODE>
+int launch_cgu(const CguArgs<R>& a, const Tiling& t, int nrhs, hipStream_t s) {
+    const dim3 grid(t.nsx, (a.Lt + a.TB - 1) / a.TB, nrhs);
+    if (a.ptab)
+        hipLaunchKernelGGL((cg_update_kernel<R, MODE, true>), grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((cg_update_kernel<R, MODE, false>), grid, dim3(256), 0, s, a);
+    QF_LAUNCHED();
+    return QF_OK;
+}
+
+inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Workspace carve-up of the PCG solvers (identical in size query and solve).
+struct CgWork {
+    void* R[2];
+    void* P[2];
+    void* Z;
+    void* OP;
+    void* XLO;   // mixed only (complex64)
+    void* BLO;   // mixed only (complex64)
+    double* rzp;
+    double* rrp;
+    double* pap;
+    CgState* st;
+    double* normb;
+    double* rp64;   // mixed only
+    size_t bytes;
+};
+
+CgWork carve(void* base, int prec, int mixed, int nrhs, int L, int Lt, const Tiling& t, const Tiling& t64) {
+    CgWork w;
+    const size_t N = (size_t)L * L * Lt;
+    const size_t vb = al256((size_t)nrhs * N * (prec == QF_PREC_F64 ? 16 : 8));
+    char* p = static_cast<char*>(base);
+    size_t off = 0;
+    auto take = [&](size_t n) { void* r = p ? p + off : nullptr; off += al256(n); return r; };
+    w.R[0] = take(vb); w.R[1] = take(vb); w.P[0] = take(vb); w.P[1] = take(vb); w.Z = take(vb); w.OP = take(vb);
+    w.XLO = mixed ? take(vb) : nullptr;
+    w.BLO = mixed ? take(vb) : nullptr;
+    w.rzp = (double*)take(sizeof(double) * 2 * nrhs * t.nB);
+    w.rrp = (double*)take(sizeof(double) * 2 * nrhs * t.nB);
+    w.pap = (double*)take(sizeof(double) * nrhs * t.nA);
+    w.st = (CgState*)take(sizeof(CgState) * 2 * nrhs);
+    w.normb = (double*)take(sizeof(double) * nrhs);
+    w.rp64 = mixed ? (double*)take(sizeof(double) * nrhs * t64.nA) : nullptr;
+    w.bytes = off;
+    return w;
+}
+
+template <typename R>
+int pcg_iterate(const R* phi, long long phi_cs, const cplx<R>* b_in, cplx<R>* x, int nrhs, int L, int Lt, const Hop<R>& hp,
+                const R* ptab, double rtol, int max_iter, const CgWork& w, const Tiling& t, hipStream_t s,
+                // mixed-precision reliable updates (R = float only)
+                int ru_period, const double* phi64, long long phi64_cs, const double2* b_hi, double2* x_hi, const Hop<double>* hp64,
+                const Tiling* t64, double* err_out, double* iters_out) {
+    const int Vs = L * L;
+    const long long N = (long long)Vs * Lt;
+    CguArgs<R> u{};
+    u.vs = N; u.Vs = Vs; u.Lt = Lt; u.TB = t.TB; u.ptab = ptab; u.nA = t.nA;
+    MdmArgs<R> m{};
+    m.phi = phi; m.phi_cs = phi_cs; m.vs = N; m.L = L; m.Lt = Lt; m.T = t.T; m.hp = hp;
+    m.nB = t.nB; m.normb = w.normb; m.rtol = rtol; m.dot_part = w.pap; m.z = (const cplx<R>*)w.Z; m.y = (cplx<R>*)w.OP;
+    const long long ps = (long long)nrhs * t.nB;   // partial slot stride
+
+    // init: x = 0, r = b, z = P b, partials into slot 1, state slot 1
+    u.rold = b_in; u.x = x; u.rnew = (cplx<R>*)w.R[1]; u.z = (cplx<R>*)w.Z;
+    u.st_init = w.st + nrhs; u.rz_part = w.rzp + ps; u.rr_part = w.rrp + ps;
+    int rc = launch_cgu<R, CGU_INIT>(u, t, nrhs, s);
+    if (rc) return rc;
+
+    for (int i = 0; i < max_iter; ++i) {
+        const int cur = i & 1, prv = cur ^ 1;
+        m.iter = i;
+        m.rz_part = w.rzp + prv * ps; m.rr_part = w.rrp + prv * ps;
+        m.st_prev = w.st + prv * nrhs; m.st_out = w.st + cur * nrhs;
+        m.pold = (const cplx<R>*)w.P[prv]; m.pnew = (cplx<R>*)w.P[cur];
+        rc = launch_mdm<R, MDM_CG>(m, t.by, nrhs, s);
+        if (rc) return rc;
+        u.rold = (const cplx<R>*)w.R[prv]; u.op = (const cplx<R>*)w.OP; u.p = (const cplx<R>*)w.P[cur];
+        u.x = x; u.rnew = (cplx<R>*)w.R[cur]; u.z = (cplx<R>*)w.Z;
+        u.pap = w.pap; u.st = w.st + cur * nrhs;
+        u.rz_part = w.rzp + cur * ps; u.rr_part = w.rrp + cur * ps;
+        rc = launch_cgu<R, CGU_ITER>(u, t, nrhs, s);
+        if (rc) return rc;
+        if (ru_period > 0 && (i + 1) % ru_period == 0 && i + 1 < max_iter) {
+            // reliable update: x_hi += x_lo; r = b - A64 x_hi (FP64); z = P r
+            const long long n = (long long)nrhs * N;
+            hipLaunchKernelGGL(flush_x_kernel, dim3(std::min<long long>((n + 255) / 256, 8192)), dim3(256), 0, s, x_hi, (float2*)x, n);
+            QF_LAUNCHED();
+            MdmArgs<double> r{};
+            r.phi = phi64; r.phi_cs = phi64_cs; r.x = x_hi; r.vs = N; r.L = L; r.Lt = Lt; r.T = t64->T; r.hp = *hp64;
+            r.b = b_hi; r.r_lo = (float2*)w.R[cur]; r.dot_part = w.rp64;
+            rc = launch_mdm<double, MDM_RESID>(r, t64->by, nrhs, s);
+            if (rc) return rc;
+            CguArgs<R> f = u;
+            f.rold = (const cplx<R>*)w.R[cur];
+            rc = launch_cgu<R, CGU_REFRESH>(f, t, nrhs, s);
+            if (rc) return rc;
+        }
+    }
+    const int last = (max_iter - 1) & 1;
+    if (ru_period > 0) {
+        const long long n = (long long)nrhs * N;
+        hipLaunchKernelGGL(flush_x_kernel, dim3(std::min<long long>((n + 255) / 256, 8192)), dim3(256), 0, s, x_hi, (float2*)x, n);
+        QF_LAUNCHED();
+        MdmArgs<double> r{};
+        r.phi = phi64; r.phi_cs = phi64_cs; r.x = x_hi; r.vs = N; r.L = L; r.Lt = Lt; r.T = t64->T; r.hp = *hp64;
+        r.b = b_hi; r.r_lo = (float2*)w.R[last]; r.dot_part = w.rp64;
+        rc = launch_mdm<double, MDM_RESID>(r, t64->by, nrhs, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(cg_finalize_kernel, dim3(nrhs), dim3(64), 0, s, (const double*)w.rp64, t64->nA,
+                           (const CgState*)(w.st + last * nrhs), (const double*)w.normb, rtol, 1, err_out, iters_out);
+    } else {
+        hipLaunchKernelGGL(cg_finalize_kernel, dim3(nrhs), dim3(64), 0, s, (const double*)(w.rrp + last * ps), t.nB,
+                           (const CgState*)(w.st + last * nrhs), (const double*)w.normb, rtol, 0, err_out, iters_out);
+    }
+    QF_LAUNCHED();
+    return QF_OK;
+}
+
+// ------------------------------------------------------------ preconditioner
+// Float32 roundings of the manual band constants of
+// preconditioners_orig/precon_manual.py:37-167 (the reference stores them as
+// complex64).  edge/tail/middle per rank 0..6, corners per rank 1..5.
+const float kEdge0[] = {0.91698223f, 1.02107012f, 1.1740706f, 1.24646831f, 1.28092396f, 1.29741096f, 1.30534041f, 1.30917168f, 1.3110323f, 1.31193936f};
+const float kTail0[] = {1.31167626f, 1.31074941f, 1.30885756f, 1.30498219f, 1.29699361f, 1.28042281f, 1.24588311f, 1.17343438f, 1.02052784f};
+const float kEdge1[] = {0.475227f, 0.69804072f, 0.80315554f, 0.85303694f, 0.87684089f, 0.88826168f, 0.89376724f, 0.89643389f, 0.89773124f, 0.89836514f, 0.89867532f, 0.89882821f};
+const float kTail1[] = {0.89785033f, 0.89652652f, 0.89382064f, 0.88826007f, 0.87676316f, 0.85285789f, 0.80287397f, 0.69771057f};
+const float kEdge2[] = {0.32525402f, 0.47824016f, 0.550632f, 0.58508581f, 0.60157132f, 0.60949975f, 0.61333305f, 0.61519265f};
+const float kTail2[] = {0.61507541f, 0.61318511f, 0.60930747f, 0.60131705f, 0.58475631f, 0.5502165f, 0.47778079f};
+const float kEdge3[] = {0.22288805f, 0.32804558f, 0.3779504f, 0.40176818f, 0.41319385f, 0.4187026f, 0.42137364f, 0.42267111f};
+const float kTail3[] = {0.42132217f, 0.41861743f, 0.41305307f, 0.40155935f, 0.37766892f, 0.3276957f};
+const float kEdge4[] = {0.15300766f, 0.22541766f, 0.25988016f, 0.27637246f, 0.28430402f, 0.28813654f, 0.29000109f, 0.29090756f, 0.291352f};
+const float kTail4[] = {0.29132634f, 0.29086968f, 0.28994057f, 0.28805143f, 0.28417516f, 0.27618375f, 0.25962788f, 0.22511749f};
+const float kEdge5[] = {0.10518534f, 0.15511395f, 0.17894313f, 0.190377f, 0.19588688f, 0.1985613f, 0.19986567f, 0.20050026f, 0.20081173f, 0.20096488f, 0.20104057f};
+const float kTail5[] = {0.20103745f, 0.20095888f, 0.20079921f, 0.20047785f, 0.19982809f, 0.19850214f, 0.19579838f, 0.19023941f, 0.17874525f, 0.15486649f};
+const float kEdge6[] = {0.10691644f, 0.12341493f, 0.13135427f, 0.13519259f, 0.13706173f, 0.13796897f, 0.13841458f};
+const float kTail6[] = {0.13839549f, 0.13793865f, 0.13701266f, 0.13511762f, 0.13124515f, 0.12326149f, 0.10670882f};
+const float kMid[] = {1.312381281f, 0.8989771f, 0.616974f, 0.42391914f, 0.29178312f, 0.20111492f, 0.13884714f};
+const float kCorner[] = {-0.47505581f, -0.32495788f, -0.22264314f, -0.15278356f, -0.10501137f};
+
+struct Band {
+    const float* e;
+    int ne;
+    const float* t;
+    int nt;
+};
+const Band kBand[7] = {{kEdge0, 10, kTail0, 9},  {kEdge1, 12, kTail1, 8}, {kEdge2, 8, kTail2, 7}, {kEdge3, 8, kTail3, 6},
+                       {kEdge4, 9, kTail4, 8},   {kEdge5, 11, kTail5, 10}, {kEdge6, 7, kTail6, 7}};
+
+}  // namespace
+
+// =================================================================== C ABI
+extern "C" {
+
+int qf_version(void) { return QF_ABI_VERSION; }
+
+const char* qf_last_error(void) { return g_err.c_str(); }
+
+int qf_set_tuning(int prec, int by, int t, int tb) {
+    if (prec != QF_PREC_F32 && prec != QF_PREC_F64) return fail(QF_ERR_INVALID, "prec %d", prec);
+    if (by != 0 && (by % 2 || by < 2 || by > 6)) return fail(QF_ERR_INVALID, "patch height %d not in {0,2,4,6}", by);
+    if (t < 0 || tb < 0) return fail(QF_ERR_INVALID, "negative tile size");
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[prec].by = by;
+    g_tune[prec].t = t;
+    g_tune[prec].tb = tb;
+    return QF_OK;
+}
+
+int qf_get_tiling(int prec, int nrhs, int L, int Ltau, int* out4) {
+    int rc = check_lattice(L, Ltau, nrhs);
+    if (rc) return rc;
+    Tiling t = tiling(prec, nrhs, L, Ltau);
+    out4[0] = t.by; out4[1] = t.T; out4[2] = t.TB; out4[3] = t.nA;
+    return QF_OK;
+}
+
+int qf_precon_table(int Ltau, double* table) {
+    if (!table) return fail(QF_ERR_INVALID, "null table");
+    if (Ltau < 26) return fail(QF_ERR_LTAU, "the manual preconditioner needs Ltau >= 26 (got %d)", Ltau);
+    std::memset(table, 0, sizeof(double) * 13 * (size_t)Ltau);
+    for (int tau = 0; tau < Ltau; ++tau) {
+        const Band& b = kBand[0];
+        float v = tau < b.ne ? b.e[tau] : tau >= Ltau - b.nt ? b.t[tau - (Ltau - b.nt)] : kMid[0];
+        table[tau * 13 + 6] = v;
+    }
+    for (int r = 1; r <= 6; ++r) {
+        const Band& b = kBand[r];
+        const int n = Ltau - r - (r == 6 ? 1 : 0);
+        const int row0 = r == 6 ? 1 : 0;
+        for (int k = 0; k < n; ++k) {
+            const float v = k < b.ne ? b.e[k] : k >= n - b.nt ? b.t[k - (n - b.nt)] : kMid[r];
+            table[(row0 + k) * 13 + 6 + r] += v;
+            table[(row0 + k + r) * 13 + 6 - r] += v;
+        }
+    }
+    for (int r = 1; r <= 5; ++r) {
+        table[0 * 13 + 6 - r] += kCorner[r - 1];
+        table[(Ltau - r) * 13 + 6 + r] += kCorner[r - 1];
+    }
+    return QF_OK;
+}
+
+int qf_precon_table_from_csr(const long long* crow, const long long* col, const double* val_re, const double* val_im, long long nrows,
+                             int Vs, int Ltau, double* table) {
+    if (!crow || !col || !val_re || !table) return fail(QF_ERR_INVALID, "null argument");
+    if (Vs < 1 || Ltau < 1 || nrows != (long long)Vs * Ltau) return fail(QF_ERR_INVALID, "CSR has %lld rows, expected Vs*Ltau", nrows);
+    std::vector<char> seen((size_t)13 * Ltau, 0);
+    std::memset(table, 0, sizeof(double) * 13 * (size_t)Ltau);
+    // pass 1: site 0 of every slice defines the tau stencil
+    for (int tau = 0; tau < Ltau; ++tau) {
+        const long long row = (long long)tau * Vs;
+        for (long long k = crow[row]; k < crow[row + 1]; ++k) {
+            const long long c = col[k];
+            if (c < 0 || c >= nrows) return fail(QF_ERR_PRECON, "column %lld out of range", c);
+            if (c % Vs != 0) return fail(QF_ERR_PRECON, "row %lld couples different sites: not P_tau (x) I", row);
+            int o = (int)(c / Vs) - tau;
+            if (o > Ltau / 2) o -= Ltau;
+            if (o < -Ltau / 2) o += Ltau;
+            if (o < -6 || o > 6) return fail(QF_ERR_PRECON, "tau offset %d outside the 13-band", o);
+            if (val_im && val_im[k] != 0.0) return fail(QF_ERR_PRECON, "imaginary preconditioner entry");
+            table[tau * 13 + 6 + o] += val_re[k];
+            seen[tau * 13 + 6 + o] = 1;
+        }
+    }
+    // pass 2: every other site repeats the same stencil
+    for (long long row = 0; row < nrows; ++row) {
+        const int tau = (int)(row / Vs), s = (int)(row % Vs);
+        double acc[13] = {0};
+        for (long long k = crow[row]; k < crow[row + 1]; ++k) {
+            const long long c = col[k];
+            if (c < 0 || c >= nrows || c % Vs != s) return fail(QF_ERR_PRECON, "row %lld: entry off the site diagonal", row);
+            int o = (int)(c / Vs) - tau;
+            if (o > Ltau / 2) o -= Ltau;
+            if (o < -Ltau / 2) o += Ltau;
+            if (o < -6 || o > 6) return fail(QF_ERR_PRECON, "row %lld: tau offset %d outside the band", row, o);
+            acc[o + 6] += val_re[k];
+        }
+        for (int o = 0; o < 13; ++o)
+            if (acc[o] != table[tau * 13 + o]) return fail(QF_ERR_PRECON, "row %lld: stencil differs from site 0 of its slice", row);
+    }
+    return QF_OK;
+}
+
+int qf_apply(int op, int prec, const void* phi, long long phi_cs, const void* x, void* y, int nrhs, int L, int Ltau, double dtau, double t,
+             void* stream) {
+    int rc = check_lattice(L, Ltau, nrhs);
+    if (rc) return rc;
+    if (!phi || !x || !y) return fail(QF_ERR_INVALID, "null pointer");
+    if (x == y) return fail(QF_ERR_INVALID, "x and y must not alias");
+    hipStream_t s = (hipStream_t)stream;
+    const Tiling tl = tiling(prec, nrhs, L, Ltau);
+    const long long N = (long long)L * L * Ltau;
+    auto run = [&](auto tag) -> int {
+        using R = decltype(tag);
+        if (op == QF_OP_MDAGM) {
+            MdmArgs<R> a{};
+            a.phi = (const R*)phi; a.phi_cs = phi_cs; a.x = (const cplx<R>*)x; a.y = (cplx<R>*)y; a.vs = N;
+            a.L = L; a.Lt = Ltau; a.T = tl.T; a.hp = make_hop<R>(dtau, t);
+            return launch_mdm<R, MDM_PLAIN>(a, tl.by, nrhs, s);
+        }
+        OpArgs<R> a{};
+        a.phi = (const R*)phi; a.phi_cs = phi_cs; a.x = (const cplx<R>*)x; a.y = (cplx<R>*)y; a.vs = N;
+        a.L = L; a.Lt = Ltau; a.T = tl.T; a.hp = make_hop<R>(dtau, t); a.nfac = 7;
+        if (op >= QF_OP_BPART0 && op <= QF_OP_BPART0 + 7) {
+            a.nfac = op - QF_OP_BPART0;
+            return launch_op<R, OP_B>(a, tl.by, nrhs, s);
+        }
+        if (op == QF_OP_M) return launch_op<R, OP_M>(a, tl.by, nrhs, s);
+        if (op == QF_OP_MDAG) return launch_op<R, OP_MDAG>(a, tl.by, nrhs, s);
+        if (op == QF_OP_B) return launch_op<R, OP_B>(a, tl.by, nrhs, s);
+        return fail(QF_ERR_INVALID, "unknown operator %d", op);
+    };
+    if (prec == QF_PREC_F32) return run(float{});
+    if (prec == QF_PREC_F64) return run(double{});
+    return fail(QF_ERR_INVALID, "prec %d", prec);
+}
+
+int qf_mhm_vec(int prec, const void* phi, const void* x, void* y, int bs, int L, int Ltau, double dtau, void* stream) {
+    return qf_apply(QF_OP_MDAGM, prec, phi, (long long)2 * L * L * Ltau, x, y, bs, L, Ltau, dtau, 1.0, stream);
+}
+
+int qf_mh_vec(int prec, const void* phi, const void* x, void* y, int bs, int L, int Ltau, double dtau, void* stream) {
+    return qf_apply(QF_OP_MDAG, prec, phi, (long long)2 * L * L * Ltau, x, y, bs, L, Ltau, dtau, 1.0, stream);
+}
+
+int qf_precon_vec(int prec, const void* ptab, const void* r, void* z, int nrhs, int Vs, int Ltau, void* stream) {
+    if (!ptab || !r || !z) return fail(QF_ERR_INVALID, "null pointer");
+    if (r == z) return fail(QF_ERR_INVALID, "r and z must not alias");
+    if (Vs < 1 || Ltau < 1 || nrhs < 1 || nrhs > 65535) return fail(QF_ERR_INVALID, "bad sizes");
+    hipStream_t s = (hipStream_t)stream;
+    Tiling t{};
+    t.nsx = (Vs + 255) / 256;
+    const long long units = (long long)nrhs * Ltau;
+    t.TB = std::min<int>(Ltau, (int)std::max(2LL, std::min(64LL, units * t.nsx / 1024)));
+    auto run = [&](auto tag) -> int {
+        using R = decltype(tag);
+        CguArgs<R> a{};
+        a.rold = (const cplx<R>*)r; a.z = (cplx<R>*)z; a.vs = (long long)Vs * Ltau; a.Vs = Vs; a.Lt = Ltau; a.TB = t.TB;
+        a.ptab = (const R*)ptab;
+        return launch_cgu<R, CGU_PRECON>(a, t, nrhs, s);
+    };
+    if (prec == QF_PREC_F32) return run(float{});
+    if (prec == QF_PREC_F64) return run(double{});
+    return fail(QF_ERR_INVALID, "prec %d", prec);
+}
+
+size_t qf_pcg_workspace_bytes(int prec, int mixed, int nrhs, int L, int Ltau) {
+    if (check_lattice(L, Ltau, nrhs)) return 0;
+    const Tiling t = tiling(mixed ? QF_PREC_F32 : prec, nrhs, L, Ltau);
+    const Tiling t64 = tiling(QF_PREC_F64, nrhs, L, Ltau);
+    return carve(nullptr, mixed ? QF_PREC_F32 : prec, mixed, nrhs, L, Ltau, t, t64).bytes;
+}
+
+int qf_pcg_solve(int prec, const void* phi, long long phi_cs, const void* b, void* x, int nrhs, int L, int Ltau, double dtau, double t,
+                 const void* ptab, double rtol, int max_iter, void* work, size_t work_bytes, double* err_out, double* iters_out,
+                 void* stream) {
+    int rc = check_lattice(L, Ltau, nrhs);
+    if (rc) return rc;
+    if (!phi || !b || !x || !work || !err_out || !iters_out) return fail(QF_ERR_INVALID, "null pointer");
+    if (max_iter < 1) return fail(QF_ERR_INVALID, "max_iter=%d must be >= 1", max_iter);
+    const Tiling tl = tiling(prec, nrhs, L, Ltau);
+    const Tiling t64 = tiling(QF_PREC_F64, nrhs, L, Ltau);
+    const CgWork w = carve(work, prec, 0, nrhs, L, Ltau, tl, t64);
+    if (work_bytes < w.bytes) return fail(QF_ERR_INVALID, "workspace of %zu bytes, need %zu", work_bytes, w.bytes);
+    hipStream_t s = (hipStream_t)stream;
+    if (prec == QF_PREC_F32)
+        return pcg_iterate<float>((const float*)phi, phi_cs, (const float2*)b, (float2*)x, nrhs, L, Ltau, make_hop<float>(dtau, t),
+                                  (const float*)ptab, rtol, max_iter, w, tl, s, 0, nullptr, 0, nullptr, nullptr, nullptr, nullptr, err_out,
+                                  iters_out);
+    if (prec == QF_PREC_F64)
+        return pcg_iterate<double>((const double*)phi, phi_cs, (const double2*)b, (double2*)x, nrhs, L, Ltau, make_hop<double>(dtau, t),
+                                   (const double*)ptab, rtol, max_iter, w, tl, s, 0, nullptr, 0, nullptr, nullptr, nullptr, nullptr,
+                                   err_out, iters_out);
+    return fail(QF_ERR_INVALID, "prec %d", prec);
+}
+
+int qf_pcg_mixed_solve(const double* phi64, const float* phi32, long long phi_cs, const void* b, void* x, int nrhs, int L, int Ltau,
+                       double dtau, double t, const float* ptab32, double rtol, int max_iter, int ru_period, void* work,
+                       size_t work_bytes, double* err_out, double* iters_out, void* stream) {
+    int rc = check_lattice(L, Ltau, nrhs);
+    if (rc) return rc;
+    if (!phi64 || !phi32 || !b || !x || !work || !err_out || !iters_out) return fail(QF_ERR_INVALID, "null pointer");
+    if (max_iter < 1) return fail(QF_ERR_INVALID, "max_iter=%d must be >= 1", max_iter);
+    if (ru_period < 1) return fail(QF_ERR_INVALID, "reliable-update period %d must be >= 1", ru_period);
+    const Tiling tl = tiling(QF_PREC_F32, nrhs, L, Ltau);
+    const Tiling t64 = tiling(QF_PREC_F64, nrhs, L, Ltau);
+    const CgWork w = carve(work, QF_PREC_F32, 1, nrhs, L, Ltau, tl, t64);
+    if (work_bytes < w.bytes) return fail(QF_ERR_INVALID, "workspace of %zu bytes, need %zu", work_bytes, w.bytes);
+    hipStream_t s = (hipStream_t)stream;
+    const long long n = (long long)nrhs * L * L * Ltau;
+    const dim3 g1(std::min<long long>((n + 255) / 256, 8192));
+    hipLaunchKernelGGL(demote_kernel, g1, dim3(256), 0, s, (const double2*)b, (float2*)w.BLO, n);
+    QF_LAUNCHED();
+    hipLaunchKernelGGL(zero_kernel, g1, dim3(256), 0, s, (double*)x, 2 * n);
+    QF_LAUNCHED();
+    const Hop<double> hp64 = make_hop<double>(dtau, t);
+    return pcg_iterate<float>(phi32, phi_cs, (const float2*)w.BLO, (float2*)w.XLO, nrhs, L, Ltau, make_hop<float>(dtau, t), ptab32, rtol,
+                              max_iter, w, tl, s, ru_period, phi64, phi_cs, (const double2*)b, (double2*)x, &hp64, &t64, err_out,
+                              iters_out);
+}
+
+int qf_fermion_force(int prec, const void* phi, long long phi_cs, const void* xi, void* Ft, int nrhs, int L, int Ltau, double dtau,
+                     double t, void* stream) {
+    int rc = check_lattice(L, Ltau, nrhs);
+    if (rc) return rc;
+    if (!phi || !xi || !Ft) return fail(QF_ERR_INVALID, "null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    const Tiling tl = tiling(prec, nrhs, L, Ltau);
+    const long long N = (long long)L * L * Ltau;
+    auto run = [&](auto tag) -> int {
+        using R = decltype(tag);
+        ForceArgs<R> a{};
+        a.phi = (const R*)phi; a.phi_cs = phi_cs; a.xi = (const cplx<R>*)xi; a.vs = N; a.Ft = (R*)Ft; a.f_cs = 2 * N;
+        a.L = L; a.Lt = Ltau; a.T = tl.T; a.hp = make_hop<R>(dtau, t);
+        switch (tl.by) {
+            case 2: return launch_force_by<R, 2>(a, nrhs, s);
+            case 4: return launch_force_by<R, 4>(a, nrhs, s);
+            case 6: return launch_force_by<R, 6>(a, nrhs, s);
+        }
+        return fail(QF_ERR_UNSUPPORTED, "patch height %d", tl.by);
+    };
+    if (prec == QF_PREC_F32) return run(float{});
+    if (prec == QF_PREC_F64) return run(double{});
+    return fail(QF_ERR_INVALID, "prec %d", prec);
+}
+
+int qf_boson_force(const double* x, double* Fp, double* Ft, int nrhs, int L, int Ltau, double cJ, double cK, int compact, void* stream) {
+    int rc = check_lattice(L, Ltau, nrhs);
+    if (rc) return rc;
+    if (!x || !Fp || !Ft) return fail(QF_ERR_INVALID, "null pointer");
+    BosonArgs a{};
+    a.x = (double*)x; a.Fp = Fp; a.Ft = Ft; a.cs = 2LL * L * L * Ltau; a.L = L; a.Lt = Ltau; a.nrhs = nrhs;
+    a.cJ = cJ; a.cK = cK; a.compact = compact; a.nsub = 1; a.m = 1.0;
+    const long long ns = (long long)nrhs * L * L * Ltau;
+    hipLaunchKernelGGL(boson_force_kernel, dim3(std::min<long long>((ns + 255) / 256, 16384)), dim3(256), 0, (hipStream_t)stream, a, 0);
+    QF_LAUNCHED();
+    return QF_OK;
+}
+
+int qf_leapfrog_boson(double* x, double* p, double* Fp, double* Ft, const double* dt, const double* tau_mask, double m, int nsub, int nrhs,
+                      int L, int Ltau, double cJ, double cK, int compact, void* stream) {
+    int rc = check_lattice(L, Ltau, nrhs);
+    if (rc) return rc;
+    if (!x || !p || !Fp || !Ft || !dt) return fail(QF_ERR_INVALID, "null pointer");
+    if (nsub < 1) return fail(QF_ERR_INVALID, "nsub=%d", nsub);
+    BosonArgs a{};
+    a.x = x; a.p = p; a.Fp = Fp; a.Ft = Ft; a.dt = dt; a.tau_mask = tau_mask; a.cs = 2LL * L * L * Ltau; a.L = L; a.Lt = Ltau;
+    a.nrhs = nrhs; a.cJ = cJ; a.cK = cK; a.m = m; a.compact = compact; a.nsub = nsub;
+    hipStream_t s = (hipStream_t)stream;
+    const long long ns = (long long)nrhs * L * L * Ltau, nr = 2 * ns;
+    for (int k = 0; k < nsub; ++k) {
+        hipLaunchKernelGGL(boson_kick_drift_kernel, dim3(std::min<long long>((nr + 255) / 256, 16384)), dim3(256), 0, s, a);
+        QF_LAUNCHED();
+        hipLaunchKernelGGL(boson_force_kernel, dim3(std::min<long long>((ns + 255) / 256, 16384)), dim3(256), 0, s, a, 1);
+        QF_LAUNCHED();
+    }
+    return QF_OK;
+}
+
+int qf_momentum_kick(double* p, const double* F, const double* dt, double scale, const double* tau_mask, int nrhs, int L, int Ltau,
+                     void* stream) {
+    if (!p || !F || !dt) return fail(QF_ERR_INVALID, "null pointer");
+    if (nrhs < 1 || L < 1 || Ltau < 1) return fail(QF_ERR_INVALID, "bad sizes");
+    const long long cs = 2LL * L * L * Ltau, n = nrhs * cs;
+    hipLaunchKernelGGL(momentum_kick_kernel, dim3(std::min<long long>((n + 255) / 256, 16384)), dim3(256), 0, (hipStream_t)stream, p, F, dt,
+                       scale, tau_mask, cs, 2 * L * L, n);
+    QF_LAUNCHED();
+    return QF_OK;
+}
+
+size_t qf_hamiltonian_workspace_bytes(int nrhs) { return sizeof(double) * 4 * 64 * (size_t)std::max(nrhs, 1); }
+
+int qf_hamiltonian(int prec, const double* x, const double* p, const void* psi, const void* xi, double* H, double* parts, int nrhs, int L,
+                   int Ltau, double cJ, double cK, int compact, double m, void* work, size_t work_bytes, void* stream) {
+    int rc = check_lattice(L, Ltau, nrhs);
+    if (rc) return rc;
+    if (!x || !H || !work) return fail(QF_ERR_INVALID, "null pointer");
+    if ((psi == nullptr) != (xi == nullptr)) return fail(QF_ERR_INVALID, "psi and xi must both be given or both be null");
+    if (work_bytes < qf_hamiltonian_workspace_bytes(nrhs)) return fail(QF_ERR_INVALID, "workspace too small");
+    hipStream_t s = (hipStream_t)stream;
+    const int nblk = 64;
+    const long long cs = 2LL * L * L * Ltau, vs = (long long)L * L * Ltau;
+    double* part = (double*)work;
+    if (prec == QF_PREC_F64)
+        hipLaunchKernelGGL((hamiltonian_partial_kernel<double>), dim3(nblk, nrhs), dim3(256), 0, s, x, p, (const double2*)psi,
+                           (const double2*)xi, cs, vs, L, Ltau, cJ, cK, compact, m, part);
+    else if (prec == QF_PREC_F32)
+        hipLaunchKernelGGL((hamiltonian_partial_kernel<float>), dim3(nblk, nrhs), dim3(256), 0, s, x, p, (const float2*)psi,
+                           (const float2*)xi, cs, vs, L, Ltau, cJ, cK, compact, m, part);
+    else
+        return fail(QF_ERR_INVALID, "prec %d", prec);
+    QF_LAUNCHED();
+    hipLaunchKernelGGL(hamiltonian_finalize_kernel, dim3(nrhs), dim3(64), 0, s, (const double*)part, nblk, H, parts);
+    QF_LAUNCHED();
+    return QF_OK;
+}
+
+}  // extern "C"

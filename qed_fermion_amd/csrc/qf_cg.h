@@ -1,0 +1,184 @@
+// qf_cg.h -- fused PCG update step and preconditioner application.
+//
+// The preconditioner of the reference (preconditioners_orig/precon_manual.py:
+// 15-175, loaded at hmc_sampler_batch.py:461-547) is P = P_tau (x) I_Vs with
+// P_tau a symmetric 13-band matrix plus antiperiodic corners.  Here it is a
+// cyclic 13-point tau stencil: z_tau = sum_{o=-6..6} c[tau][o] r_{(tau+o) mod Nt}
+// (the corners (0, Nt-r) and (Nt-r, 0) are the cyclic offsets -r / +r of rows
+// 0 and Nt-r, which the open band never uses).
+//
+// One thread owns one site and marches in tau with a 13-slice register window,
+// so every r value is read from memory about once per block instead of 13x.
+//
+// Update step B_i of preconditioned_cg_fast_test (hmc_sampler_batch.py:797-844):
+//   alpha = rz_old / Re(p^dag Op);  x += alpha p;  r -= alpha Op;  z = P r;
+//   partial sums of Re(r^dag z) and |r|^2 per block (FP64).
+// r is double buffered (rold -> rnew) because z needs the new r of the
+// neighbouring slices, which it recomputes on the fly from rold and Op.
+#pragma once
+#include "qf_reduce.h"
+
+namespace qf {
+
+enum { CGU_INIT = 0, CGU_ITER = 1, CGU_REFRESH = 2, CGU_PRECON = 3 };
+
+template <typename R>
+struct CguArgs {
+    const cplx<R>* rold;   // INIT: b; ITER: r_{i-1}; REFRESH/PRECON: r
+    const cplx<R>* op;     // ITER: M^dag M p
+    const cplx<R>* p;      // ITER
+    cplx<R>* x;            // ITER: x += alpha p; INIT: x = 0
+    cplx<R>* rnew;         // INIT/ITER: new r
+    cplx<R>* z;            // z = P r
+    long long vs;
+    int Vs, Lt, TB;
+    const R* ptab;         // [Lt][13] or nullptr (no preconditioner)
+    const double* pap;     // ITER: Re(p^dag Op) partials [nrhs][nA]
+    int nA;
+    const CgState* st;     // ITER: state written by step A of this iteration
+    CgState* st_init;      // INIT: state slot read by step A_0
+    double* rz_part;       // [nrhs][nblk]
+    double* rr_part;
+};
+
+template <typename R, int MODE, bool PRE>
+__global__ void __launch_bounds__(256) cg_update_kernel(CguArgs<R> A) {
+    __shared__ double red[64];
+    const int rho = blockIdx.z;
+    const int Vs = A.Vs, Lt = A.Lt;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = s < Vs;
+    const int t0 = blockIdx.y * A.TB;
+    const int t1 = min(t0 + A.TB, Lt);
+    const long long vo = rho * A.vs;
+    const int blk = blockIdx.x + gridDim.x * blockIdx.y;
+    R alpha = R(0);
+
+    if constexpr (MODE == CGU_ITER) {
+        const CgState S = A.st[rho];
+        if (S.act == 0.0) return;   // converged chain: x and r stay frozen
+        double sums[1];
+        const double* src[1] = {A.pap + (long long)rho * A.nA};
+        sum_partials<1>(src, A.nA, sums, red);
+        alpha = (R)(S.rz / sums[0]);
+    }
+    if constexpr (MODE == CGU_INIT) {
+        if (blk == 0 && threadIdx.x == 0) {
+            CgState o; o.rz = 1.0; o.act = 1.0; o.err = 1.0; o.iters = 0.0;
+            A.st_init[rho] = o;
+        }
+    }
+
+    const cplx<R>* rold = A.rold + vo;
+    const cplx<R>* op = A.op + vo;
+    auto rn = [&](int tau) -> cplx<R> {
+        const long long i = (long long)tau * Vs + s;
+        cplx<R> r = rold[i];
+        if constexpr (MODE == CGU_ITER) {
+            const cplx<R> o = op[i];
+            r = mk<R>(r.x - alpha * o.x, r.y - alpha * o.y);
+        }
+        return r;
+    };
+
+    double rr = 0.0, rz = 0.0;
+    cplx<R> w[13];
+#pragma unroll
+    for (int k = 0; k < 13; ++k) w[k] = mk<R>(0, 0);
+    if (valid) {
+        if constexpr (PRE) {
+#pragma unroll
+            for (int k = 1; k < 13; ++k) w[k] = rn(((t0 - 7 + k) % Lt + Lt) % Lt);
+        }
+        for (int tau = t0; tau < t1; ++tau) {
+            cplx<R> r, zz;
+            if constexpr (PRE) {
+#pragma unroll
+                for (int k = 0; k < 12; ++k) w[k] = w[k + 1];
+                w[12] = rn((tau + 6) % Lt);
+                const R* c = A.ptab + tau * 13;
+                zz = mk<R>(0, 0);
+#pragma unroll
+                for (int k = 0; k < 13; ++k) zz = mk<R>(zz.x + c[k] * w[k].x, zz.y + c[k] * w[k].y);
+                r = w[6];
+            } else {
+                r = rn(tau);
+                zz = r;
+            }
+            const long long i = vo + (long long)tau * Vs + s;
+            if constexpr (MODE == CGU_ITER) {
+                const cplx<R> pp = A.p[i];
+                cplx<R> xx = A.x[i];
+                A.x[i] = mk<R>(xx.x + alpha * pp.x, xx.y + alpha * pp.y);
+            }
+            if constexpr (MODE == CGU_INIT) A.x[i] = mk<R>(0, 0);
+            if constexpr (MODE == CGU_INIT || MODE == CGU_ITER) A.rnew[i] = r;
+            A.z[i] = zz;
+            rr += (double)r.x * r.x + (double)r.y * r.y;
+            rz += (double)r.x * zz.x + (double)r.y * zz.y;
+        }
+    }
+    if constexpr (MODE != CGU_PRECON) {
+        double v[2] = {rz, rr};
+        block_sum<2>(v, red);
+        if (threadIdx.x == 0) {
+            const long long nblk = (long long)gridDim.x * gridDim.y;
+            A.rz_part[rho * nblk + blk] = v[0];
+            A.rr_part[rho * nblk + blk] = v[1];
+        }
+    }
+}
+
+// Scalars after the last iteration (the reference returns the error of the
+// last iteration and, without graphs, the iteration counter, 828-846).
+// One block per RHS.  If true_res, rr partials hold |b - A x|^2 in FP64 and the
+// error is reported from them even for converged chains.
+__global__ void cg_finalize_kernel(const double* rr_part, int n, const CgState* st_prev, const double* normb, double rtol,
+                                   int true_res, double* err_out, double* iters_out) {
+    __shared__ double red[64];
+    const int rho = blockIdx.x;
+    double sums[1];
+    const double* src[1] = {rr_part + (long long)rho * n};
+    sum_partials<1>(src, n, sums, red);
+    if (threadIdx.x == 0) {
+        const CgState S = st_prev[rho];
+        const double nb = normb[rho];
+        double err, it = S.iters;
+        if (true_res) {
+            err = nb > 0.0 ? sqrt(sums[0]) / nb : 0.0;
+            if (S.act != 0.0 && err > rtol) it += 1.0;
+        } else if (S.act == 0.0) {
+            err = S.err;
+        } else {
+            err = nb > 0.0 ? sqrt(sums[0]) / nb : 0.0;
+            if (err > rtol) it += 1.0;
+        }
+        err_out[rho] = err;
+        iters_out[rho] = it;
+    }
+}
+
+// x_hi += x_lo; x_lo = 0 (reliable update of the mixed-precision solver).
+__global__ void flush_x_kernel(double2* __restrict__ xhi, float2* __restrict__ xlo, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const float2 l = xlo[i];
+        double2 h = xhi[i];
+        h.x += (double)l.x;
+        h.y += (double)l.y;
+        xhi[i] = h;
+        xlo[i] = make_float2(0.f, 0.f);
+    }
+}
+
+__global__ void demote_kernel(const double2* __restrict__ src, float2* __restrict__ dst, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double2 v = src[i];
+        dst[i] = make_float2((float)v.x, (float)v.y);
+    }
+}
+
+__global__ void zero_kernel(double* __restrict__ p, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+}  // namespace qf

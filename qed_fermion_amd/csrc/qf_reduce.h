@@ -1,0 +1,58 @@
+// qf_reduce.h -- deterministic FP64 block reductions.
+//
+// Every reduction in this library has a fixed summation tree (lane-strided
+// partial sums, then xor-shuffle butterflies, then waves in index order), so
+// the same input gives bit-identical sums in every block and on every replay.
+#pragma once
+#include "qf_common.h"
+
+namespace qf {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sum over the block of up to NV values per thread.  red: LDS scratch of
+// NV * 16 doubles.  Result valid in every thread.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* red) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) red[k * 16 + w] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double s = 0.0;
+        for (int i = 0; i < nw; ++i) s += red[k * 16 + i];
+        v[k] = s;
+    }
+    __syncthreads();
+}
+
+// Sum n doubles (fixed order) using the first wave; result broadcast through
+// LDS to the whole block.  Called by all threads.
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double* const (&src)[NV], int n, double (&out)[NV], double* red) {
+    const int lane = threadIdx.x & 63;
+    if (threadIdx.x < 64) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double s = 0.0;
+            for (int i = lane; i < n; i += 64) s += src[k][i];
+            s = wave_sum(s);
+            if (lane == 0) red[k] = s;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[k] = red[k];
+    __syncthreads();
+}
+
+}  // namespace qf

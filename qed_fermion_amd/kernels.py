@@ -1,0 +1,230 @@
+"""Torch-tensor front end of the HIP kernels in libqf_hip.so.
+
+Tensors must be contiguous CUDA (HIP) tensors in the reference's permuted
+layouts (hmc_sampler_batch.py:2166): ``phi`` ``[nchain, Ltau*L*L*2]`` real and
+vectors ``[nrhs, Ltau*L*L]`` complex.  complex64/float32 selects the FP32
+kernels, complex128/float64 the FP64 ones.  Every launch goes on torch's
+current stream, so these calls can be captured with ``torch.cuda.graph``.
+Outputs and workspaces are allocated with torch (never inside the library).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import QfError, call, ptr, stream_handle
+
+_PREC = {torch.complex64: _lib.QF_PREC_F32, torch.complex128: _lib.QF_PREC_F64,
+         torch.float32: _lib.QF_PREC_F32, torch.float64: _lib.QF_PREC_F64}
+_REAL = {torch.complex64: torch.float32, torch.complex128: torch.float64}
+
+
+def _prec(t):
+    try:
+        return _PREC[t.dtype]
+    except KeyError:
+        raise QfError(_lib.QF_ERR_INVALID, f"unsupported dtype {t.dtype}") from None
+
+
+def _check_dev(*ts):
+    for t in ts:
+        if t is None:
+            continue
+        if not t.is_cuda:
+            raise QfError(_lib.QF_ERR_INVALID, "tensor is not on the GPU; the HIP path has no CPU fallback")
+        if not t.is_contiguous():
+            raise QfError(_lib.QF_ERR_INVALID, "tensor must be contiguous")
+
+
+def _phi_stride(phi, nrhs, N):
+    """Chain stride of phi: 0 when one field is shared by all right-hand sides."""
+    nphi = phi.numel() // (2 * N)
+    if phi.numel() != nphi * 2 * N:
+        raise QfError(_lib.QF_ERR_INVALID, f"phi has {phi.numel()} reals, not a multiple of 2*N={2 * N}")
+    if nphi == nrhs:
+        return 2 * N
+    if nphi == 1:
+        return 0
+    raise QfError(_lib.QF_ERR_INVALID, f"phi holds {nphi} fields for {nrhs} right-hand sides")
+
+
+def apply(op, phi, x, L, Ltau, dtau=0.1, t=1.0, out=None):
+    """y = op(phi) x; op in {'M', 'Mdag', 'MdagM', 'B'}."""
+    code = {"M": _lib.QF_OP_M, "Mdag": _lib.QF_OP_MDAG, "MdagM": _lib.QF_OP_MDAGM, "B": _lib.QF_OP_B}[op]
+    N = L * L * Ltau
+    x2 = x.reshape(-1, N)
+    _check_dev(phi, x2)
+    if phi.dtype != _REAL[x.dtype]:
+        raise QfError(_lib.QF_ERR_INVALID, f"phi dtype {phi.dtype} does not match vector dtype {x.dtype}")
+    nrhs = x2.shape[0]
+    y = torch.empty_like(x2) if out is None else out.reshape(-1, N)
+    call("qf_apply", code, _prec(x), ptr(phi), _phi_stride(phi, nrhs, N), ptr(x2), ptr(y), nrhs, L, Ltau, float(dtau),
+         float(t), stream_handle())
+    return y.view(x.shape)
+
+
+def apply_partial_B(phi, x, L, Ltau, dtau, nfac, t=1.0):
+    """y_tau = Q_nfac x_tau: the first nfac factors of B_tau = D4 D3 D2 D1 D2 D3 D4 applied per slice."""
+    if not 0 <= nfac <= 7:
+        raise QfError(_lib.QF_ERR_INVALID, f"nfac={nfac}")
+    N = L * L * Ltau
+    x2 = x.reshape(-1, N)
+    _check_dev(phi, x2)
+    y = torch.empty_like(x2)
+    call("qf_apply", 16 + int(nfac), _prec(x), ptr(phi), _phi_stride(phi, x2.shape[0], N), ptr(x2), ptr(y), x2.shape[0], L,
+         Ltau, float(dtau), float(t), stream_handle())
+    return y.view(x.shape)
+
+
+def precon_table(Ltau, dtype=torch.float64, device="cuda"):
+    """The manual preconditioner (precon_manual.py:15-175) as a [Ltau, 13] band table."""
+    tab = np.zeros((Ltau, 13), dtype=np.float64)
+    call("qf_precon_table", Ltau, tab.ctypes.data_as(ctypes.c_void_p))
+    return torch.from_numpy(tab).to(device=device, dtype=dtype)
+
+
+def precon_table_from_csr(P_csr, Vs, Ltau):
+    """Verify that a torch CSR matrix is P_tau (x) I_Vs with a 13-band tau stencil; return its table (float64, CPU)."""
+    P = P_csr.to("cpu")
+    crow = P.crow_indices().to(torch.int64).numpy()
+    col = P.col_indices().to(torch.int64).numpy()
+    vals = P.values().detach().clone()
+    if vals.is_complex():
+        ri = torch.view_as_real(vals).to(torch.float64).numpy()
+        vr, vi = np.ascontiguousarray(ri[:, 0]), np.ascontiguousarray(ri[:, 1])
+    else:
+        vr, vi = np.ascontiguousarray(vals.to(torch.float64).numpy()), None
+    crow = np.ascontiguousarray(crow)
+    col = np.ascontiguousarray(col)
+    tab = np.zeros((Ltau, 13), dtype=np.float64)
+    c_p = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+    call("qf_precon_table_from_csr", c_p(crow), c_p(col), c_p(vr), c_p(vi), P.shape[0], Vs, Ltau, c_p(tab))
+    return torch.from_numpy(tab)
+
+
+def apply_precon(ptab, r, Vs, Ltau, out=None):
+    N = Vs * Ltau
+    r2 = r.reshape(-1, N)
+    _check_dev(ptab, r2)
+    tab = ptab.to(_REAL[r.dtype]).contiguous()
+    z = torch.empty_like(r2) if out is None else out.reshape(-1, N)
+    call("qf_precon_vec", _prec(r), ptr(tab), ptr(r2), ptr(z), r2.shape[0], Vs, Ltau, stream_handle())
+    return z.view(r.shape)
+
+
+_work_cache = {}
+
+
+def _workspace(key, nbytes, device):
+    buf = _work_cache.get(key)
+    if buf is None or buf.numel() < nbytes:
+        buf = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=device)
+        _work_cache[key] = buf
+    return buf
+
+
+def pcg_workspace_bytes(prec, mixed, nrhs, L, Ltau):
+    return _lib.lib().qf_pcg_workspace_bytes(prec, int(bool(mixed)), nrhs, L, Ltau)
+
+
+def pcg_solve(phi, b, L, Ltau, dtau=0.1, t=1.0, ptab=None, rtol=1e-9, max_iter=400, x=None, work=None, err=None, iters=None):
+    """Batched PCG (preconditioned_cg_fast_test recurrence).  Returns (x, iters, err) as device tensors."""
+    N = L * L * Ltau
+    b2 = b.reshape(-1, N)
+    _check_dev(phi, b2, ptab)
+    nrhs = b2.shape[0]
+    prec = _prec(b)
+    x = torch.empty_like(b2) if x is None else x.reshape(-1, N)
+    nbytes = pcg_workspace_bytes(prec, 0, nrhs, L, Ltau)
+    if nbytes == 0:
+        raise QfError(_lib.QF_ERR_INVALID, _lib.lib().qf_last_error().decode())
+    work = _workspace(("pcg", prec, nrhs, L, Ltau, b.device), nbytes, b.device) if work is None else work
+    err = torch.empty(nrhs, dtype=torch.float64, device=b.device) if err is None else err
+    iters = torch.empty(nrhs, dtype=torch.float64, device=b.device) if iters is None else iters
+    tab = ptab.to(_REAL[b.dtype]).contiguous() if ptab is not None else None
+    call("qf_pcg_solve", prec, ptr(phi), _phi_stride(phi, nrhs, N), ptr(b2), ptr(x), nrhs, L, Ltau, float(dtau), float(t),
+         ptr(tab), float(rtol), int(max_iter), ptr(work), work.numel(), ptr(err), ptr(iters), stream_handle())
+    return x.view(b.shape), iters, err
+
+
+def pcg_mixed_solve(phi64, b, L, Ltau, dtau=0.1, t=1.0, ptab=None, rtol=1e-10, max_iter=400, ru_period=50, phi32=None,
+                    x=None, work=None, err=None, iters=None):
+    """Mixed-precision PCG: complex64 iterations with FP64 reliable updates every ``ru_period``
+    iterations.  b complex128, phi64 float64.  Returns (x complex128, iters, err_true)."""
+    N = L * L * Ltau
+    b2 = b.reshape(-1, N)
+    if b2.dtype != torch.complex128 or phi64.dtype != torch.float64:
+        raise QfError(_lib.QF_ERR_INVALID, "mixed solve takes complex128 b and float64 phi")
+    _check_dev(phi64, b2, ptab)
+    nrhs = b2.shape[0]
+    phi32 = phi64.to(torch.float32) if phi32 is None else phi32
+    x = torch.empty_like(b2) if x is None else x.reshape(-1, N)
+    nbytes = pcg_workspace_bytes(_lib.QF_PREC_F32, 1, nrhs, L, Ltau)
+    work = _workspace(("mixed", nrhs, L, Ltau, b.device), nbytes, b.device) if work is None else work
+    err = torch.empty(nrhs, dtype=torch.float64, device=b.device) if err is None else err
+    iters = torch.empty(nrhs, dtype=torch.float64, device=b.device) if iters is None else iters
+    tab = ptab.to(torch.float32).contiguous() if ptab is not None else None
+    call("qf_pcg_mixed_solve", ptr(phi64), ptr(phi32), _phi_stride(phi64, nrhs, N), ptr(b2), ptr(x), nrhs, L, Ltau,
+         float(dtau), float(t), ptr(tab), float(rtol), int(max_iter), int(ru_period), ptr(work), work.numel(), ptr(err),
+         ptr(iters), stream_handle())
+    return x.view(b.shape), iters, err
+
+
+def fermion_force(phi, xi, L, Ltau, dtau=0.1, t=1.0, out=None):
+    """Ft = -dS_f/dphi in the phi layout [nrhs, Ltau*L*L*2]."""
+    N = L * L * Ltau
+    xi2 = xi.reshape(-1, N)
+    _check_dev(phi, xi2)
+    nrhs = xi2.shape[0]
+    F = torch.empty(nrhs, 2 * N, dtype=_REAL[xi.dtype], device=xi.device) if out is None else out
+    call("qf_fermion_force", _prec(xi), ptr(phi), _phi_stride(phi, nrhs, N), ptr(xi2), ptr(F), nrhs, L, Ltau, float(dtau),
+         float(t), stream_handle())
+    return F
+
+
+def boson_force(x, L, Ltau, cJ, cK, compact=False, Fp=None, Ft=None):
+    _check_dev(x)
+    nrhs = x.numel() // (2 * L * L * Ltau)
+    Fp = torch.empty_like(x) if Fp is None else Fp
+    Ft = torch.empty_like(x) if Ft is None else Ft
+    call("qf_boson_force", ptr(x), ptr(Fp), ptr(Ft), nrhs, L, Ltau, float(cJ), float(cK), int(bool(compact)), stream_handle())
+    return Fp, Ft
+
+
+def leapfrog_boson(x, p, Fp, Ft, dt, L, Ltau, cJ, cK, compact=False, m=1.0, nsub=5, tau_mask=None):
+    """leapfrog_cmp in place on (x, p, Fp, Ft); dt float64 [nchain]."""
+    _check_dev(x, p, Fp, Ft, dt, tau_mask)
+    nrhs = x.numel() // (2 * L * L * Ltau)
+    call("qf_leapfrog_boson", ptr(x), ptr(p), ptr(Fp), ptr(Ft), ptr(dt), ptr(tau_mask), float(m), int(nsub), nrhs, L, Ltau,
+         float(cJ), float(cK), int(bool(compact)), stream_handle())
+
+
+def momentum_kick(p, F, dt, scale, L, Ltau, tau_mask=None):
+    _check_dev(p, F, dt, tau_mask)
+    nrhs = p.numel() // (2 * L * L * Ltau)
+    call("qf_momentum_kick", ptr(p), ptr(F), ptr(dt), float(scale), ptr(tau_mask), nrhs, L, Ltau, stream_handle())
+
+
+def hamiltonian(x, L, Ltau, cJ, cK, compact=False, m=1.0, p=None, psi=None, xi=None, H=None, parts=None):
+    """Per-chain H = S_tau + S_plaq + p^2/2m + Re(psi^dag xi) in FP64; returns (H, parts[nrhs, 4])."""
+    _check_dev(x, p, psi, xi)
+    nrhs = x.numel() // (2 * L * L * Ltau)
+    H = torch.empty(nrhs, dtype=torch.float64, device=x.device) if H is None else H
+    parts = torch.empty(nrhs, 4, dtype=torch.float64, device=x.device) if parts is None else parts
+    prec = _prec(psi) if psi is not None else _lib.QF_PREC_F64
+    nbytes = _lib.lib().qf_hamiltonian_workspace_bytes(nrhs)
+    work = _workspace(("ham", nrhs, x.device), nbytes, x.device)
+    call("qf_hamiltonian", prec, ptr(x), ptr(p), ptr(psi), ptr(xi), ptr(H), ptr(parts), nrhs, L, Ltau, float(cJ), float(cK),
+         int(bool(compact)), float(m), ptr(work), work.numel(), stream_handle())
+    return H, parts
+
+
+def set_tuning(prec, by=0, t=0, tb=0):
+    call("qf_set_tuning", int(prec), int(by), int(t), int(tb))
+
+
+def get_tiling(prec, nrhs, L, Ltau):
+    out = (ctypes.c_int * 4)()
+    call("qf_get_tiling", int(prec), nrhs, L, Ltau, out)
+    return {"by": out[0], "t": out[1], "tb": out[2], "nA": out[3]}

@@ -1,0 +1,76 @@
+"""CPU checks of the C-ABI library: it loads without a GPU, exports every symbol
+the header declares, and its host-side functions (preconditioner table, CSR
+verification, argument validation) behave."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "qed_fermion_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(qf_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from qed_fermion_amd import _lib
+    h = _lib.lib()
+    names = _declared()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(h, n), n
+    assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
+    assert h.qf_version() == 1
+
+
+def test_precon_table_matches_oracle():
+    from qed_fermion_amd import _lib
+    from oracle import oracle as O
+    for Lt in (26, 60, 360):
+        t = np.zeros((Lt, 13))
+        _lib.call("qf_precon_table", Lt, t.ctypes.data_as(ctypes.c_void_p))
+        assert np.array_equal(t, O.precon_table(Lt))
+    with pytest.raises(_lib.QfError) as e:
+        _lib.call("qf_precon_table", 25, np.zeros((25, 13)).ctypes.data_as(ctypes.c_void_p))
+    assert e.value.code == _lib.QF_ERR_LTAU
+
+
+def test_precon_csr_roundtrip_and_rejection():
+    import torch
+    from qed_fermion_amd import precon
+    from qed_fermion_amd._lib import QF_ERR_PRECON, QfError
+    L, Lt = 4, 30
+    d = precon.get_precon_man(L, Lt)
+    csr = precon.coo_to_csr(d, device="cpu")
+    tab = precon.table_from_csr(csr, L * L, Lt)
+    ref = precon.band_table(Lt)
+    # the CSR is complex64: values are the float32 constants, so the round trip is exact
+    assert torch.equal(tab, ref)
+    bad = d["values"].clone()
+    bad[5] += 1.0          # one site differs from the others of its slice
+    csr_bad = precon.coo_to_csr({"indices": d["indices"], "values": bad, "size": d["size"]}, device="cpu")
+    with pytest.raises(QfError) as e:
+        precon.table_from_csr(csr_bad, L * L, Lt)
+    assert e.value.code == QF_ERR_PRECON
+
+
+def test_argument_validation_without_gpu():
+    from qed_fermion_amd import _lib
+    h = _lib.lib()
+    assert h.qf_pcg_workspace_bytes(1, 0, 1, 5, 10) == 0            # odd L rejected
+    assert "even L" in h.qf_last_error().decode()
+    assert h.qf_pcg_workspace_bytes(1, 0, 2, 36, 360) > 0
+    rc = h.qf_apply(4, 0, None, 0, None, None, 1, 6, 10, 0.1, 1.0, None)
+    assert rc == _lib.QF_ERR_INVALID
+    rc = h.qf_apply(4, 0, None, 0, None, None, 1, 7, 10, 0.1, 1.0, None)
+    assert rc == _lib.QF_ERR_ODD_L
+    rc = h.qf_set_tuning(0, 3, 0, 0)
+    assert rc == _lib.QF_ERR_INVALID
+    out = (ctypes.c_int * 4)()
+    assert h.qf_get_tiling(0, 1, 36, 360, out) == 0 and out[0] in (2, 4, 6)
+    assert h.qf_get_tiling(0, 1, 66, 660, out) == 0 and out[0] == 6
