@@ -66,6 +66,16 @@ int qf_get_tiling(int prec, int nrhs, int L, int Ltau, int* out4);
 int qf_apply(int op, int prec, const void* phi, long long phi_cs, const void* x, void* y, int nrhs, int L, int Ltau, double dtau,
              double t, void* stream);
 
+/* Hop table of nchain fields: hop[chain][tau][y][x][mu] = tanh(a_g) e^{i phi_mu} (complex,
+ * 2*Ltau*L*L per chain), the per-link factor coefficients of get_M's dB matrices
+ * (hmc_sampler_batch.py:1560-1598) normalised by cosh(a_g).  qf_apply_hop applies the
+ * operators from a table instead of phi (no sincos per application; hop_cs = 0 shares
+ * one table between all right-hand sides).  No separate counterpart in the reference,
+ * whose kernels take the boson field directly. */
+int qf_hop_table(int prec, const void* phi, long long phi_cs, int nchain, int L, int Ltau, double dtau, double t, void* hop, void* stream);
+int qf_apply_hop(int op, int prec, const void* hop, long long hop_cs, const void* x, void* y, int nrhs, int L, int Ltau, double dtau, double t,
+                 void* stream);
+
 /* _C.mhm_vec(boson, x, Lx, dtau, *BLOCK_SIZE) (hmc_sampler_batch.py:772, 799): batched y = M^dag M x, t = 1. */
 int qf_mhm_vec(int prec, const void* phi, const void* x, void* y, int bs, int L, int Ltau, double dtau, void* stream);
 

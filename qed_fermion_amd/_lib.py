@@ -49,6 +49,8 @@ SIGNATURES = {
     "qf_set_tuning": (_I, [_I, _I, _I, _I]),
     "qf_get_tiling": (_I, [_I, _I, _I, _I, _P]),
     "qf_apply": (_I, [_I, _I, _P, _LL, _P, _P, _I, _I, _I, _D, _D, _P]),
+    "qf_hop_table": (_I, [_I, _P, _LL, _I, _I, _I, _D, _D, _P, _P]),
+    "qf_apply_hop": (_I, [_I, _I, _P, _LL, _P, _P, _I, _I, _I, _D, _D, _P]),
     "qf_mhm_vec": (_I, [_I, _P, _P, _P, _I, _I, _I, _D, _P]),
     "qf_mh_vec": (_I, [_I, _P, _P, _P, _I, _I, _I, _D, _P]),
     "qf_precon_vec": (_I, [_I, _P, _P, _P, _I, _I, _I, _P]),

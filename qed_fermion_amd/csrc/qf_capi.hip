@@ -109,7 +109,19 @@ Hop<R> make_hop(double dtau, double t) {
     h.sh1 = (R)std::sinh(t * dtau);
     h.ch2 = (R)std::cosh(0.5 * t * dtau);
     h.sh2 = (R)std::sinh(0.5 * t * dtau);
+    h.th1 = (R)std::tanh(t * dtau);
+    h.th2 = (R)std::tanh(0.5 * t * dtau);
+    h.cn = (R)(std::cosh(t * dtau) * std::pow(std::cosh(0.5 * t * dtau), 6));
+    h.dtau_ = dtau;
+    h.t_ = t;
     return h;
+}
+
+// Product of cosh(a_g) over the first nfac factors of D4 D3 D2 D1 D2 D3 D4.
+double cosh_prefix(int nfac, double dtau, double t) {
+    double c = 1.0;
+    for (int k = 0; k < nfac; ++k) c *= (k == 3) ? std::cosh(t * dtau) : std::cosh(0.5 * t * dtau);
+    return c;
 }
 
 int check_lattice(int L, int Lt, int nrhs) {
@@ -142,7 +154,7 @@ inline dim3 blk_of(int L, int by) { return dim3((unsigned)(((nthreads(L, by) + 6
 // ---------------------------------------------------------------- launchers
 template <typename R, int BY, int MODE>
 int launch_mdm_by(const MdmArgs<R>& a, int nrhs, hipStream_t s) {
-    const size_t lds = lds_bytes<R>(a.L, 1);
+    const size_t lds = lds_bytes<R>(a.L, 2);   // two vectors per exchange (tau pipeline)
     const dim3 grid((a.Lt + a.T - 1) / a.T, nrhs), blk = blk_of(a.L, BY);
     if (blk.x <= 512) {
         auto k = mdagm_kernel<R, BY, MODE, 512>;
@@ -230,6 +242,15 @@ int launch_cgu(const CguArgs<R>& a, const Tiling& t, int nrhs, hipStream_t s) {
 
 inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+template <typename R>
+int launch_hop_table(const R* phi, long long phi_cs, cplx<R>* hop, int nchain, int L, int Lt, double dtau, double t, hipStream_t s) {
+    const long long n = (long long)nchain * L * L * Lt;
+    hipLaunchKernelGGL((hop_table_kernel<R>), dim3(std::min<long long>((n + 255) / 256, 16384)), dim3(256), 0, s, phi, phi_cs, hop, nchain,
+                       L, Lt, std::tanh(t * dtau), std::tanh(0.5 * t * dtau));
+    QF_LAUNCHED();
+    return QF_OK;
+}
+
 // Workspace carve-up of the PCG solvers (identical in size query and solve).
 struct CgWork {
     void* R[2];
@@ -238,12 +259,16 @@ struct CgWork {
     void* OP;
     void* XLO;   // mixed only (complex64)
     void* BLO;   // mixed only (complex64)
+    void* HOP;   // hop table of the solve's field(s), [chain][tau][y][x][mu] complex
     double* rzp;
     double* rrp;
     double* pap;
     CgState* st;
     double* normb;
     double* rp64;   // mixed only
+    double* ru_rr;  // mixed only: [2][nrhs]
+    double* ru_it;  // mixed only: [2][nrhs]
+    int* ru_flag;   // mixed only: [2][nrhs]
     size_t bytes;
 };
 
@@ -257,12 +282,16 @@ CgWork carve(void* base, int prec, int mixed, int nrhs, int L, int Lt, const Til
     w.R[0] = take(vb); w.R[1] = take(vb); w.P[0] = take(vb); w.P[1] = take(vb); w.Z = take(vb); w.OP = take(vb);
     w.XLO = mixed ? take(vb) : nullptr;
     w.BLO = mixed ? take(vb) : nullptr;
+    w.HOP = take(2 * vb);
     w.rzp = (double*)take(sizeof(double) * 2 * nrhs * t.nB);
     w.rrp = (double*)take(sizeof(double) * 2 * nrhs * t.nB);
     w.pap = (double*)take(sizeof(double) * nrhs * t.nA);
     w.st = (CgState*)take(sizeof(CgState) * 2 * nrhs);
     w.normb = (double*)take(sizeof(double) * nrhs);
     w.rp64 = mixed ? (double*)take(sizeof(double) * nrhs * t64.nA) : nullptr;
+    w.ru_rr = mixed ? (double*)take(sizeof(double) * 2 * nrhs) : nullptr;
+    w.ru_it = mixed ? (double*)take(sizeof(double) * 2 * nrhs) : nullptr;
+    w.ru_flag = mixed ? (int*)take(sizeof(int) * 2 * nrhs) : nullptr;
     w.bytes = off;
     return w;
 }
@@ -281,6 +310,12 @@ int pcg_iterate(const R* phi, long long phi_cs, const cplx<R>* b_in, cplx<R>* x,
     m.phi = phi; m.phi_cs = phi_cs; m.vs = N; m.L = L; m.Lt = Lt; m.T = t.T; m.hp = hp;
     m.nB = t.nB; m.normb = w.normb; m.rtol = rtol; m.dot_part = w.pap; m.z = (const cplx<R>*)w.Z; m.y = (cplx<R>*)w.OP;
     const long long ps = (long long)nrhs * t.nB;   // partial slot stride
+    // hop table of the field(s): the iterations read it instead of recomputing sincos
+    const int nchain = phi_cs ? nrhs : 1;
+    int rc0 = launch_hop_table<R>(phi, phi_cs, (cplx<R>*)w.HOP, nchain, L, Lt, hp.dtau_, hp.t_, s);
+    if (rc0) return rc0;
+    m.hop = (const cplx<R>*)w.HOP;
+    m.hop_cs = phi_cs ? 2 * N : 0;
 
     // init: x = 0, r = b, z = P b, partials into slot 1, state slot 1
     u.rold = b_in; u.x = x; u.rnew = (cplx<R>*)w.R[1]; u.z = (cplx<R>*)w.Z;
@@ -288,45 +323,77 @@ int pcg_iterate(const R* phi, long long phi_cs, const cplx<R>* b_in, cplx<R>* x,
     int rc = launch_cgu<R, CGU_INIT>(u, t, nrhs, s);
     if (rc) return rc;
 
+    // Mixed precision: reliable-update checks every kRuCheck iterations (see
+    // ru_decide_flush_kernel); the check after iteration i uses parity c & 1.
+    constexpr int kRuCheck = 5;
+    const bool mixed = ru_period > 0;
+    const long long n_el = (long long)nrhs * N;
+    const int ru_blocks = (int)std::max(1LL, std::min((N + 1023) / 1024, 512LL));
+    int check = 0;
+    auto reliable_update = [&](int iters_done, int slot, int force, int cur) -> int {
+        RuArgs a{};
+        const int c = check & 1, pc = c ^ 1;
+        a.xhi = x_hi; a.xlo = (float2*)x; a.N = N;
+        a.rr_part = w.rrp + slot * ps; a.nB = t.nB; a.rp64 = w.rp64; a.nA64 = t64->nA;
+        a.rr_prev = w.ru_rr + pc * nrhs; a.rr_out = w.ru_rr + c * nrhs;
+        a.it_prev = w.ru_it + pc * nrhs; a.it_out = w.ru_it + c * nrhs;
+        a.flag_prev = w.ru_flag + pc * nrhs; a.flag_out = w.ru_flag + c * nrhs;
+        a.st = w.st + cur * nrhs; a.iter = iters_done; a.ru_max = ru_period; a.delta2 = 1e-2; a.force = force;
+        hipLaunchKernelGGL(ru_decide_flush_kernel, dim3(ru_blocks, nrhs), dim3(256), 0, s, a);
+        QF_LAUNCHED();
+        MdmArgs<double> r{};
+        r.phi = phi64; r.phi_cs = phi64_cs; r.x = x_hi; r.vs = N; r.L = L; r.Lt = Lt; r.T = t64->T; r.hp = *hp64;
+        r.b = b_hi; r.r_lo = (float2*)w.R[cur]; r.dot_part = w.rp64; r.ru_flag = w.ru_flag + c * nrhs;
+        int rc2 = launch_mdm<double, MDM_RESID>(r, t64->by, nrhs, s);
+        if (rc2) return rc2;
+        ++check;
+        return QF_OK;
+    };
+    (void)n_el;
+
     for (int i = 0; i < max_iter; ++i) {
         const int cur = i & 1, prv = cur ^ 1;
         m.iter = i;
+        // mixed precision: a chain may only stop on a true (FP64) residual,
+        // i.e. right after a reliable update (stop_flag of the last check);
+        // the FP32 iterated residual can drift below the true one.
+        m.may_stop = 1;
+        m.stop_flag = nullptr;
+        if (mixed && i > 0) {
+            if (i % kRuCheck == 0) m.stop_flag = w.ru_flag + ((check - 1) & 1) * nrhs;
+            else m.may_stop = 0;
+        }
         m.rz_part = w.rzp + prv * ps; m.rr_part = w.rrp + prv * ps;
         m.st_prev = w.st + prv * nrhs; m.st_out = w.st + cur * nrhs;
         m.pold = (const cplx<R>*)w.P[prv]; m.pnew = (cplx<R>*)w.P[cur];
         rc = launch_mdm<R, MDM_CG>(m, t.by, nrhs, s);
         if (rc) return rc;
+        if (mixed && i == 0) {
+            hipLaunchKernelGGL(ru_init_kernel, dim3((nrhs + 63) / 64), dim3(64), 0, s, (const double*)w.normb, w.ru_rr + nrhs,
+                               w.ru_it + nrhs, w.ru_flag + nrhs, nrhs);
+            QF_LAUNCHED();
+        }
         u.rold = (const cplx<R>*)w.R[prv]; u.op = (const cplx<R>*)w.OP; u.p = (const cplx<R>*)w.P[cur];
         u.x = x; u.rnew = (cplx<R>*)w.R[cur]; u.z = (cplx<R>*)w.Z;
         u.pap = w.pap; u.st = w.st + cur * nrhs;
         u.rz_part = w.rzp + cur * ps; u.rr_part = w.rrp + cur * ps;
+        u.ru_flag = nullptr;
         rc = launch_cgu<R, CGU_ITER>(u, t, nrhs, s);
         if (rc) return rc;
-        if (ru_period > 0 && (i + 1) % ru_period == 0 && i + 1 < max_iter) {
-            // reliable update: x_hi += x_lo; r = b - A64 x_hi (FP64); z = P r
-            const long long n = (long long)nrhs * N;
-            hipLaunchKernelGGL(flush_x_kernel, dim3(std::min<long long>((n + 255) / 256, 8192)), dim3(256), 0, s, x_hi, (float2*)x, n);
-            QF_LAUNCHED();
-            MdmArgs<double> r{};
-            r.phi = phi64; r.phi_cs = phi64_cs; r.x = x_hi; r.vs = N; r.L = L; r.Lt = Lt; r.T = t64->T; r.hp = *hp64;
-            r.b = b_hi; r.r_lo = (float2*)w.R[cur]; r.dot_part = w.rp64;
-            rc = launch_mdm<double, MDM_RESID>(r, t64->by, nrhs, s);
+        if (mixed && (i + 1) % kRuCheck == 0 && i + 1 < max_iter) {
+            // reliable update where needed: x_hi += x_lo; r = b - A64 x_hi (FP64); z = P r
+            rc = reliable_update(i + 1, cur, 0, cur);
             if (rc) return rc;
             CguArgs<R> f = u;
             f.rold = (const cplx<R>*)w.R[cur];
+            f.ru_flag = w.ru_flag + ((check - 1) & 1) * nrhs;
             rc = launch_cgu<R, CGU_REFRESH>(f, t, nrhs, s);
             if (rc) return rc;
         }
     }
     const int last = (max_iter - 1) & 1;
-    if (ru_period > 0) {
-        const long long n = (long long)nrhs * N;
-        hipLaunchKernelGGL(flush_x_kernel, dim3(std::min<long long>((n + 255) / 256, 8192)), dim3(256), 0, s, x_hi, (float2*)x, n);
-        QF_LAUNCHED();
-        MdmArgs<double> r{};
-        r.phi = phi64; r.phi_cs = phi64_cs; r.x = x_hi; r.vs = N; r.L = L; r.Lt = Lt; r.T = t64->T; r.hp = *hp64;
-        r.b = b_hi; r.r_lo = (float2*)w.R[last]; r.dot_part = w.rp64;
-        rc = launch_mdm<double, MDM_RESID>(r, t64->by, nrhs, s);
+    if (mixed) {
+        rc = reliable_update(max_iter, last, 1, last);
         if (rc) return rc;
         hipLaunchKernelGGL(cg_finalize_kernel, dim3(nrhs), dim3(64), 0, s, (const double*)w.rp64, t64->nA,
                            (const CgState*)(w.st + last * nrhs), (const double*)w.normb, rtol, 1, err_out, iters_out);
@@ -482,11 +549,52 @@ int qf_apply(int op, int prec, const void* phi, long long phi_cs, const void* x,
         }
         OpArgs<R> a{};
         a.phi = (const R*)phi; a.phi_cs = phi_cs; a.x = (const cplx<R>*)x; a.y = (cplx<R>*)y; a.vs = N;
-        a.L = L; a.Lt = Ltau; a.T = tl.T; a.hp = make_hop<R>(dtau, t); a.nfac = 7;
+        a.L = L; a.Lt = Ltau; a.T = tl.T; a.hp = make_hop<R>(dtau, t); a.nfac = 7; a.cpart = a.hp.cn;
         if (op >= QF_OP_BPART0 && op <= QF_OP_BPART0 + 7) {
             a.nfac = op - QF_OP_BPART0;
+            a.cpart = (R)cosh_prefix(a.nfac, dtau, t);
             return launch_op<R, OP_B>(a, tl.by, nrhs, s);
         }
+        if (op == QF_OP_M) return launch_op<R, OP_M>(a, tl.by, nrhs, s);
+        if (op == QF_OP_MDAG) return launch_op<R, OP_MDAG>(a, tl.by, nrhs, s);
+        if (op == QF_OP_B) return launch_op<R, OP_B>(a, tl.by, nrhs, s);
+        return fail(QF_ERR_INVALID, "unknown operator %d", op);
+    };
+    if (prec == QF_PREC_F32) return run(float{});
+    if (prec == QF_PREC_F64) return run(double{});
+    return fail(QF_ERR_INVALID, "prec %d", prec);
+}
+
+int qf_hop_table(int prec, const void* phi, long long phi_cs, int nchain, int L, int Ltau, double dtau, double t, void* hop, void* stream) {
+    int rc = check_lattice(L, Ltau, nchain);
+    if (rc) return rc;
+    if (!phi || !hop) return fail(QF_ERR_INVALID, "null pointer");
+    if (prec == QF_PREC_F32) return launch_hop_table<float>((const float*)phi, phi_cs, (float2*)hop, nchain, L, Ltau, dtau, t, (hipStream_t)stream);
+    if (prec == QF_PREC_F64)
+        return launch_hop_table<double>((const double*)phi, phi_cs, (double2*)hop, nchain, L, Ltau, dtau, t, (hipStream_t)stream);
+    return fail(QF_ERR_INVALID, "prec %d", prec);
+}
+
+int qf_apply_hop(int op, int prec, const void* hop, long long hop_cs, const void* x, void* y, int nrhs, int L, int Ltau, double dtau, double t,
+                 void* stream) {
+    int rc = check_lattice(L, Ltau, nrhs);
+    if (rc) return rc;
+    if (!hop || !x || !y) return fail(QF_ERR_INVALID, "null pointer");
+    if (x == y) return fail(QF_ERR_INVALID, "x and y must not alias");
+    hipStream_t s = (hipStream_t)stream;
+    const Tiling tl = tiling(prec, nrhs, L, Ltau);
+    const long long N = (long long)L * L * Ltau;
+    auto run = [&](auto tag) -> int {
+        using R = decltype(tag);
+        if (op == QF_OP_MDAGM) {
+            MdmArgs<R> a{};
+            a.hop = (const cplx<R>*)hop; a.hop_cs = hop_cs; a.x = (const cplx<R>*)x; a.y = (cplx<R>*)y; a.vs = N;
+            a.L = L; a.Lt = Ltau; a.T = tl.T; a.hp = make_hop<R>(dtau, t);
+            return launch_mdm<R, MDM_PLAIN>(a, tl.by, nrhs, s);
+        }
+        OpArgs<R> a{};
+        a.hop = (const cplx<R>*)hop; a.hop_cs = hop_cs; a.x = (const cplx<R>*)x; a.y = (cplx<R>*)y; a.vs = N;
+        a.L = L; a.Lt = Ltau; a.T = tl.T; a.hp = make_hop<R>(dtau, t); a.nfac = 7; a.cpart = a.hp.cn;
         if (op == QF_OP_M) return launch_op<R, OP_M>(a, tl.by, nrhs, s);
         if (op == QF_OP_MDAG) return launch_op<R, OP_MDAG>(a, tl.by, nrhs, s);
         if (op == QF_OP_B) return launch_op<R, OP_B>(a, tl.by, nrhs, s);

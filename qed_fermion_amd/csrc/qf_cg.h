@@ -39,6 +39,7 @@ struct CguArgs {
     CgState* st_init;      // INIT: state slot read by step A_0
     double* rz_part;       // [nrhs][nblk]
     double* rr_part;
+    const int* ru_flag;    // REFRESH: skip chains whose reliable-update flag is 0 (nullptr: never skip)
 };
 
 template <typename R, int MODE, bool PRE>
@@ -61,6 +62,9 @@ __global__ void __launch_bounds__(256) cg_update_kernel(CguArgs<R> A) {
         const double* src[1] = {A.pap + (long long)rho * A.nA};
         sum_partials<1>(src, A.nA, sums, red);
         alpha = (R)(S.rz / sums[0]);
+    }
+    if constexpr (MODE == CGU_REFRESH) {
+        if (A.ru_flag != nullptr && A.ru_flag[rho] == 0) return;
     }
     if constexpr (MODE == CGU_INIT) {
         if (blk == 0 && threadIdx.x == 0) {
@@ -155,6 +159,77 @@ __global__ void cg_finalize_kernel(const double* rr_part, int n, const CgState* 
         }
         err_out[rho] = err;
         iters_out[rho] = it;
+    }
+}
+
+// Adaptive reliable update of the mixed-precision solver (checked every few
+// iterations, entirely on the device so the solve stays graph-capturable):
+// a chain updates when its iterated residual has dropped by delta since the
+// last true residual, or ru_max iterations have passed, or at the end (force).
+// The update flushes x_hi += x_lo, x_lo = 0 here; the FP64 residual kernel and
+// the refresh kernel that follow read the per-chain flag and exit if it is 0.
+// State is parity double-buffered per check (prev -> out) because every block
+// reads it while block 0 of each chain writes the next value.
+struct RuArgs {
+    double2* xhi;
+    float2* xlo;
+    long long N;               // elements per chain
+    const double* rr_part;     // iterated |r|^2 partials of the last update step, [nrhs][nB]
+    int nB;
+    const double* rp64;        // |b - A x_hi|^2 partials of the last FP64 residual, [nrhs][nA64]
+    int nA64;
+    const double* rr_prev;     // |r_true|^2 at the last reliable update (prev parity)
+    double* rr_out;
+    const double* it_prev;     // iteration of the last reliable update
+    double* it_out;
+    const int* flag_prev;      // 1 if the previous check performed an update
+    int* flag_out;
+    const CgState* st;         // active mask
+    int iter;                  // iterations completed
+    int ru_max;
+    double delta2;
+    int force;
+};
+
+__global__ void __launch_bounds__(256) ru_decide_flush_kernel(RuArgs A) {
+    __shared__ double red[64];
+    const int rho = blockIdx.y;
+    const bool prev_updated = A.flag_prev[rho] != 0;
+    double s0[1], s1[1];
+    const double* src0[1] = {A.rr_part + (long long)rho * A.nB};
+    const double* src1[1] = {A.rp64 + (long long)rho * A.nA64};
+    sum_partials<1>(src0, A.nB, s0, red);
+    sum_partials<1>(src1, prev_updated ? A.nA64 : 0, s1, red);   // true residual of the last update
+    const double rr_now = s0[0];
+    const double rr_last = prev_updated ? s1[0] : A.rr_prev[rho];
+    const double since = (double)A.iter - A.it_prev[rho];
+    const bool active = A.st[rho].act != 0.0;
+    const bool doit = A.force || (active && (since >= A.ru_max || rr_now < A.delta2 * rr_last));
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        A.flag_out[rho] = doit ? 1 : 0;
+        A.rr_out[rho] = rr_last;
+        A.it_out[rho] = doit ? (double)A.iter : A.it_prev[rho];
+    }
+    if (!doit) return;
+    double2* xh = A.xhi + rho * A.N;
+    float2* xl = A.xlo + rho * A.N;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < A.N; i += (long long)gridDim.x * blockDim.x) {
+        const float2 l = xl[i];
+        double2 h = xh[i];
+        h.x += (double)l.x;
+        h.y += (double)l.y;
+        xh[i] = h;
+        xl[i] = make_float2(0.f, 0.f);
+    }
+}
+
+// Reliable-update state at the start: |r_true|^2 = |b|^2 at iteration 0.
+__global__ void ru_init_kernel(const double* normb, double* rr, double* it, int* flag, int nrhs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nrhs) {
+        rr[i] = normb[i] * normb[i];
+        it[i] = 0.0;
+        flag[i] = 0;
     }
 }
 

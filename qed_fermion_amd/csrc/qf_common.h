@@ -37,14 +37,37 @@ template <typename R> __device__ __forceinline__ cplx<R> pair_updc(R c, cplx<R> 
     return mk<R>(c * v.x + (h.x * w.x + h.y * w.y), c * v.y + (h.x * w.y - h.y * w.x));
 }
 
+// FP32: hardware sin/cos (v_sin_f32/v_cos_f32 after a multiply by 1/2pi),
+// absolute error ~1e-6 for |phi| <= pi -- the FP32 operator is only ever
+// used as the inner operator of a solve or at the reference's complex64
+// accuracy.  FP64: accurate sincos.
 template <typename R> __device__ __forceinline__ void qsincos(R a, R* s, R* c);
-template <> __device__ __forceinline__ void qsincos<float>(float a, float* s, float* c) { sincosf(a, s, c); }
+template <> __device__ __forceinline__ void qsincos<float>(float a, float* s, float* c) { __sincosf(a, s, c); }
 template <> __device__ __forceinline__ void qsincos<double>(double a, double* s, double* c) { sincos(a, s, c); }
 
+// Work-group barrier for LDS exchanges only: waits for this wave's LDS
+// operations, then s_barrier.  Unlike __syncthreads() it does not wait for
+// outstanding global loads, so loads issued for the next slice stay in flight
+// across the exchange barriers of the current one.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// v + h*w (normalised pair update: 4 FMAs)
+template <typename R> __device__ __forceinline__ cplx<R> pair_add(cplx<R> v, cplx<R> h, cplx<R> w) {
+    return mk<R>(fma(-h.y, w.y, fma(h.x, w.x, v.x)), fma(h.y, w.x, fma(h.x, w.y, v.y)));
+}
+// v + conj(h)*w
+template <typename R> __device__ __forceinline__ cplx<R> pair_addc(cplx<R> v, cplx<R> h, cplx<R> w) {
+    return mk<R>(fma(h.y, w.y, fma(h.x, w.x, v.x)), fma(-h.y, w.x, fma(h.x, w.y, v.y)));
+}
+
 // Trotter factor constants: a_1 = t*dtau (group 1), a_{2,3,4} = t*dtau/2
-// (hmc_sampler_batch.py:1508-1550).  sh1/ch1 for group 1, sh2/ch2 otherwise.
+// (hmc_sampler_batch.py:1508-1550).  ch/sh: cosh/sinh of a_1 (suffix 1) and of
+// a_{2,3,4} (suffix 2).  Normalised form: D_g = cosh(a_g) (I + tanh(a_g) K_g),
+// so B_tau = cn * prod_g (I + th_g K_g) with cn = ch1 * ch2^6.
 template <typename R> struct Hop {
     R ch1, sh1, ch2, sh2;
+    R th1, th2, cn;
+    double dtau_, t_;   // the parameters they were built from
 };
 
 // Per-RHS scalar state of the batched PCG (parity double-buffered).

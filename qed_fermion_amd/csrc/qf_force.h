@@ -101,16 +101,16 @@ __global__ void __launch_bounds__(NT) force_kernel(ForceArgs<R> A) {
     cplx<R> X[2 * BY], Xn[2 * BY], Av[2 * BY], Wv[2 * BY];
 #pragma unroll
     for (int k = 0; k < 2 * BY; ++k) X[k] = Xn[k] = Av[k] = Wv[k] = mk<R>(0, 0);
-    if (g.valid) load_patch<R, BY>(X, xi + (long long)t0 * Vs, g);
+    // every lane loads (padding lanes mirror patch (0,0) and write identical
+    // exchange values); only valid lanes store
+    load_patch<R, BY>(X, xi + (long long)t0 * Vs, g);
     for (int tau = t0; tau < t1; ++tau) {
         const int tn = (tau + 1 == Lt) ? 0 : tau + 1;
-        if (g.valid) {
-            load_coef<R, BY>(cf, phi + (long long)tau * 2 * Vs, g, A.hp);
-            load_patch<R, BY>(Xn, xi + (long long)tn * Vs, g);
-        }
+        load_coef<R, BY, false>(cf, phi + (long long)tau * 2 * Vs, g, A.hp);
+        load_patch<R, BY>(Xn, xi + (long long)tn * Vs, g);
 #pragma unroll
         for (int k = 0; k < 2 * BY; ++k) Wv[k] = X[k];
-        apply_B<R, BY>(Wv, cf, A.hp, lds, g, phase);   // W = Q_7(xi) = B xi
+        apply_B<R, BY, false>(Wv, cf, A.hp, lds, g, phase);   // W = Q_7(xi) = B xi
         axpy_patch<R, BY>(Av, Wv, (R)sgnD(tau, Lt), Xn);   // a = B xi + sD xi_{tau+1}
         R F[2 * BY][2];
 #pragma unroll

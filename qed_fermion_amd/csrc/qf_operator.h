@@ -40,12 +40,14 @@ template <typename R>
 struct MdmArgs {
     const R* phi;
     long long phi_cs;
+    const cplx<R>* hop;     // optional hop table (replaces phi): [chain][tau][y][x][mu]
+    long long hop_cs;       // complex per chain (0 = shared)
     const cplx<R>* x;
     cplx<R>* y;
     long long vs;
     int L, Lt, T;
     Hop<R> hp;
-    double* dot_part;       // [nrhs][nblk]: Re(x^dag y) over the run (MODE 0, optional; MODE 1)
+    double* dot_part;       // [nrhs][nblk]: ||(Mx)_tau||^2 = Re(x^dag M^dag M x) over the run (optional)
     // MODE 1 (CG step A): x_tau := z_tau + beta p_tau, written to pnew
     const cplx<R>* z;
     const cplx<R>* pold;
@@ -58,6 +60,9 @@ struct MdmArgs {
     double* normb;
     int iter;
     double rtol;
+    int may_stop;           // 0: the partials are an iterated (not true) residual; do not deactivate
+    const int* stop_flag;   // if set: per-chain may_stop (1 right after a reliable update)
+    const int* ru_flag;     // MODE 2: skip chains whose reliable-update flag is 0
     // MODE 2 (true residual in FP64): r = b - M^dag M x written as complex64
     const cplx<R>* b;
     float2* r_lo;
@@ -78,6 +83,9 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
     const R* phi = A.phi + rho * A.phi_cs;
     const long long vo = rho * A.vs;
     R beta = R(0);
+    if constexpr (MODE == MDM_RESID) {
+        if (A.ru_flag != nullptr && A.ru_flag[rho] == 0) return;   // uniform over the block
+    }
 
     if constexpr (MODE == MDM_CG) {
         // Scalars of the previous iteration, reduced in a fixed order by
@@ -93,7 +101,8 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
             act = 0.0; err = S.err; rzv = S.rz;
         } else {
             err = nb > 0.0 ? sqrt(sums[1]) / nb : 0.0;
-            act = (err > A.rtol) ? 1.0 : 0.0;
+            const int may = A.stop_flag ? A.stop_flag[rho] : A.may_stop;
+            act = (err > A.rtol || !may) ? 1.0 : 0.0;
             rzv = sums[0];
             if (A.iter > 0) {
                 bt = sums[0] / S.rz;
@@ -110,6 +119,13 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
     }
 
     const cplx<R>* x = A.x + vo;
+    const cplx<R>* hop = A.hop ? A.hop + rho * A.hop_cs : nullptr;
+    auto coef = [&](Coef<R, BY>& cf, int tau) {
+        if (hop)
+            load_coef_tab<R, BY>(cf, hop + (long long)tau * 2 * Vs, g);
+        else
+            load_coef<R, BY>(cf, phi + (long long)tau * 2 * Vs, g, A.hp);
+    };
     auto ldx = [&](int tau, cplx<R> (&v)[2 * BY]) {
         if constexpr (MODE == MDM_CG) {
             const long long so = vo + (long long)tau * Vs;
@@ -128,39 +144,53 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
         }
     };
 
+    // tau pipeline: the second B of slice tau (W = B_tau y_{tau+1}) and the
+    // first B of slice tau+1 (U = B_{tau+1} x_{tau+1}) are independent, so they
+    // run together and share every exchange barrier (7 per slice, not 14).
+    // The dot product accumulated is ||y||^2 = Re(x^dag M^dag M x) summed over
+    // the run's slices (y = Mx is in registers; no x copy needed).
     int phase = 0;
-    Coef<R, BY> cf;
-    cplx<R> X[2 * BY], Y[2 * BY], U[2 * BY], Xn[2 * BY];
+    Coef<R, BY> cfc, cfn;
+    cplx<R> X[2 * BY], Y[2 * BY], Yn[2 * BY], U[2 * BY], W[2 * BY];
 #pragma unroll
-    for (int k = 0; k < 2 * BY; ++k) X[k] = Y[k] = U[k] = Xn[k] = mk<R>(0, 0);
+    for (int k = 0; k < 2 * BY; ++k) X[k] = Y[k] = Yn[k] = U[k] = W[k] = mk<R>(0, 0);
     double dsum = 0.0;
 
+    // prologue: U = B_{t0-1} x_{t0-1}, W = B_{t0} x_{t0};  y_{t0}, y_{t0+1}
     const int tm = (t0 + Lt - 1) % Lt;
-    if (g.valid) {
-        load_coef<R, BY>(cf, phi + (long long)tm * 2 * Vs, g, A.hp);
-        ldx(tm, U);
+    const int t0n = (t0 + 1 == Lt) ? 0 : t0 + 1;
+    coef(cfn, tm);
+    coef(cfc, t0);
+    ldx(tm, U);
+    ldx(t0, X);
+    if constexpr (MODE == MDM_CG) {
+        if (g.valid) store_patch<R, BY>(A.pnew + vo + (long long)t0 * Vs, X, g);
     }
-    apply_B<R, BY>(U, cf, A.hp, lds, g, phase);
-    if (g.valid) ldx(t0, X);
-    axpy_patch<R, BY>(Y, X, (R)sgnM(t0), U);
+#pragma unroll
+    for (int k = 0; k < 2 * BY; ++k) W[k] = X[k];
+    apply_B2<R, BY>(U, cfn, W, cfc, A.hp, lds, g, phase);
+    const R cn = A.hp.cn;   // apply_B returns B v / cn
+    axpy_patch<R, BY>(Y, X, sgnM(t0) * cn, U);
+    ldx(t0n, X);
+    axpy_patch<R, BY>(Yn, X, sgnM(t0n) * cn, W);
 
     for (int tau = t0; tau < t1; ++tau) {
-        if (g.valid) load_coef<R, BY>(cf, phi + (long long)tau * 2 * Vs, g, A.hp);
-#pragma unroll
-        for (int k = 0; k < 2 * BY; ++k) U[k] = X[k];
-        apply_B<R, BY>(U, cf, A.hp, lds, g, phase);            // U = B_tau x_tau
         const int tn = (tau + 1 == Lt) ? 0 : tau + 1;
-        if (g.valid) ldx(tn, Xn);
-        axpy_patch<R, BY>(U, Xn, (R)sgnM(tn), U);               // U = y_{tau+1}
-        cplx<R> W[2 * BY];
+        const bool more = tau + 1 < t1;
 #pragma unroll
-        for (int k = 0; k < 2 * BY; ++k) W[k] = U[k];
-        apply_B<R, BY>(W, cf, A.hp, lds, g, phase);            // W = B_tau y_{tau+1}
+        for (int k = 0; k < 2 * BY; ++k) W[k] = Yn[k];
+        if (more) {
+            coef(cfn, tn);
+#pragma unroll
+            for (int k = 0; k < 2 * BY; ++k) U[k] = X[k];
+            apply_B2<R, BY>(W, cfc, U, cfn, A.hp, lds, g, phase);   // W = B_tau y_{tau+1}, U = B_{tau+1} x_{tau+1}
+        } else {
+            apply_B<R, BY>(W, cfc, A.hp, lds, g, phase);
+        }
         cplx<R> Z[2 * BY];
-        axpy_patch<R, BY>(Z, Y, (R)sgnD(tau, Lt), W);
+        axpy_patch<R, BY>(Z, Y, sgnD(tau, Lt) * cn, W);
         if (g.valid) {
             const long long so = vo + (long long)tau * Vs;
-            if constexpr (MODE == MDM_CG) store_patch<R, BY>(A.pnew + so, X, g);
             if constexpr (MODE == MDM_RESID) {
                 cplx<R> bb[2 * BY];
                 load_patch<R, BY>(bb, A.b + so, g);
@@ -178,11 +208,20 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
                 }
             } else {
                 store_patch<R, BY>(A.y + so, Z, g);
-                dsum += redot_patch<R, BY>(X, Z);
+                dsum += redot_patch<R, BY>(Y, Y);
             }
         }
+        if (more) {
+            const int tnn = (tn + 1 == Lt) ? 0 : tn + 1;
 #pragma unroll
-        for (int k = 0; k < 2 * BY; ++k) { X[k] = Xn[k]; Y[k] = U[k]; }
+            for (int k = 0; k < 2 * BY; ++k) Y[k] = Yn[k];
+            if constexpr (MODE == MDM_CG) {
+                if (g.valid) store_patch<R, BY>(A.pnew + vo + (long long)tn * Vs, X, g);
+            }
+            ldx(tnn, X);
+            axpy_patch<R, BY>(Yn, X, sgnM(tnn) * cn, U);        // y_{tau+2}
+            cfc = cfn;
+        }
     }
 
     if (A.dot_part != nullptr) {
@@ -201,12 +240,15 @@ template <typename R>
 struct OpArgs {
     const R* phi;
     long long phi_cs;
+    const cplx<R>* hop;
+    long long hop_cs;
     const cplx<R>* x;
     cplx<R>* y;
     long long vs;
     int L, Lt, T;
     Hop<R> hp;
     int nfac;   // OP_B: number of leading factors of B_tau applied (7 = all)
+    R cpart;    // OP_B: product of cosh(a_g) over those factors (cn for all 7)
 };
 
 template <typename R, int BY, int OP, int NT>
@@ -221,6 +263,13 @@ __global__ void __launch_bounds__(NT) op_kernel(OpArgs<R> A) {
     const R* phi = A.phi + rho * A.phi_cs;
     const cplx<R>* x = A.x + rho * A.vs;
     cplx<R>* y = A.y + rho * A.vs;
+    const cplx<R>* hop = A.hop ? A.hop + rho * A.hop_cs : nullptr;
+    auto coef = [&](Coef<R, BY>& c, int tau) {
+        if (hop)
+            load_coef_tab<R, BY>(c, hop + (long long)tau * 2 * Vs, g);
+        else
+            load_coef<R, BY>(c, phi + (long long)tau * 2 * Vs, g, A.hp);
+    };
     int phase = 0;
     Coef<R, BY> cf;
     cplx<R> X[2 * BY], U[2 * BY];
@@ -229,14 +278,15 @@ __global__ void __launch_bounds__(NT) op_kernel(OpArgs<R> A) {
 
     if constexpr (OP == OP_M) {
         int tp = (t0 + Lt - 1) % Lt;
-        if (g.valid) load_patch<R, BY>(U, x + (long long)tp * Vs, g);
+        // every lane loads (padding lanes mirror patch (0,0)); only valid lanes store
+        load_patch<R, BY>(U, x + (long long)tp * Vs, g);
         for (int tau = t0; tau < t1; ++tau) {
-            if (g.valid) load_coef<R, BY>(cf, phi + (long long)tp * 2 * Vs, g, A.hp);
+            coef(cf, tp);
             apply_B<R, BY>(U, cf, A.hp, lds, g, phase);        // U = B_{tau-1} x_{tau-1}
+            load_patch<R, BY>(X, x + (long long)tau * Vs, g);
             if (g.valid) {
-                load_patch<R, BY>(X, x + (long long)tau * Vs, g);
                 cplx<R> Z[2 * BY];
-                axpy_patch<R, BY>(Z, X, (R)sgnM(tau), U);
+                axpy_patch<R, BY>(Z, X, sgnM(tau) * A.hp.cn, U);
                 store_patch<R, BY>(y + (long long)tau * Vs, Z, g);
             }
 #pragma unroll
@@ -244,20 +294,18 @@ __global__ void __launch_bounds__(NT) op_kernel(OpArgs<R> A) {
             tp = tau;
         }
     } else if constexpr (OP == OP_MDAG) {
-        if (g.valid) load_patch<R, BY>(X, x + (long long)t0 * Vs, g);
+        load_patch<R, BY>(X, x + (long long)t0 * Vs, g);
         for (int tau = t0; tau < t1; ++tau) {
             const int tn = (tau + 1 == Lt) ? 0 : tau + 1;
-            if (g.valid) {
-                load_coef<R, BY>(cf, phi + (long long)tau * 2 * Vs, g, A.hp);
-                load_patch<R, BY>(U, x + (long long)tn * Vs, g);
-            }
+            coef(cf, tau);
+            load_patch<R, BY>(U, x + (long long)tn * Vs, g);
             cplx<R> Xn[2 * BY];
 #pragma unroll
             for (int k = 0; k < 2 * BY; ++k) Xn[k] = U[k];
             apply_B<R, BY>(U, cf, A.hp, lds, g, phase);        // U = B_tau x_{tau+1}
             if (g.valid) {
                 cplx<R> Z[2 * BY];
-                axpy_patch<R, BY>(Z, X, (R)sgnD(tau, Lt), U);
+                axpy_patch<R, BY>(Z, X, sgnD(tau, Lt) * A.hp.cn, U);
                 store_patch<R, BY>(y + (long long)tau * Vs, Z, g);
             }
 #pragma unroll
@@ -265,14 +313,14 @@ __global__ void __launch_bounds__(NT) op_kernel(OpArgs<R> A) {
         }
     } else {
         for (int tau = t0; tau < t1; ++tau) {
-            if (g.valid) {
-                load_coef<R, BY>(cf, phi + (long long)tau * 2 * Vs, g, A.hp);
-                load_patch<R, BY>(U, x + (long long)tau * Vs, g);
-            }
+            coef(cf, tau);
+            load_patch<R, BY>(U, x + (long long)tau * Vs, g);
             if (A.nfac == 7)
                 apply_B<R, BY>(U, cf, A.hp, lds, g, phase);
             else
                 apply_B_partial<R, BY>(U, A.nfac, cf, A.hp, lds, g, phase);
+#pragma unroll
+            for (int k = 0; k < 2 * BY; ++k) U[k] = mk<R>(A.cpart * U[k].x, A.cpart * U[k].y);
             if (g.valid) store_patch<R, BY>(y + (long long)tau * Vs, U, g);
         }
     }

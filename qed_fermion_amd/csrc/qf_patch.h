@@ -38,6 +38,8 @@ __device__ __forceinline__ Geo make_geo(int L, int tid) {
     const int na = L >> 1;
     g.nb = L / BY;
     g.valid = tid < na * g.nb;
+    // padding lanes mirror patch (0,0): they load and exchange the same values
+    // as thread 0 (identical LDS writes are benign) and never store
     const int t = g.valid ? tid : 0;
     g.a = t % na;
     g.b = t / na;
@@ -65,37 +67,88 @@ __device__ __forceinline__ cplx<R> phase(R sh, R ph) {
     return mk<R>(sh * c, sh * s);
 }
 
-// phi_s: slice base, layout [y][x][mu] (R reals).
-template <typename R, int BY>
+// phi_s: slice base, layout [y][x][mu] (R reals).  NORM: coefficients scaled
+// by tanh(a_g) for the normalised factors I + th K; otherwise by sinh(a_g).
+template <typename R, int BY, bool NORM = true>
 __device__ __forceinline__ void load_coef(Coef<R, BY>& cf, const R* __restrict__ phi_s, const Geo& g, const Hop<R>& hp) {
     const int L = g.L;
     const cplx<R>* ps = reinterpret_cast<const cplx<R>*>(phi_s);  // one (phi_x, phi_y) pair per site
     const int x0 = 2 * g.a;
-    const int xm = (x0 + L - 1) % L;
+    const int xm = x0 == 0 ? L - 1 : x0 - 1;
+    const R s1 = NORM ? hp.th1 : hp.sh1, s2 = NORM ? hp.th2 : hp.sh2;
 #pragma unroll
     for (int r = 0; r < BY; ++r) {
         const int y = g.y0 + r;
         const cplx<R> p0 = ps[y * L + x0];
         const cplx<R> p1 = ps[y * L + x0 + 1];
         const R pxm = phi_s[2 * (y * L + xm)];
-        const R sh_in = (r & 1) ? hp.sh2 : hp.sh1;   // g1 (r even) or g3 (r odd)
-        const R sh_ed = (r & 1) ? hp.sh1 : hp.sh2;
+        const R sh_in = (r & 1) ? s2 : s1;   // g1 (r even) or g3 (r odd)
+        const R sh_ed = (r & 1) ? s1 : s2;
         cf.ex[r][0] = phase<R>(sh_ed, pxm);
         cf.ex[r][1] = phase<R>(sh_in, p0.x);
         cf.ex[r][2] = phase<R>(sh_ed, p1.x);
-        cf.ey[0][r + 1] = phase<R>(hp.sh2, p0.y);
-        cf.ey[1][r + 1] = phase<R>(hp.sh2, p1.y);
+        cf.ey[0][r + 1] = phase<R>(s2, p0.y);
+        cf.ey[1][r + 1] = phase<R>(s2, p1.y);
     }
-    const int ym = (g.y0 + L - 1) % L;
+    const int ym = g.y0 == 0 ? L - 1 : g.y0 - 1;
     const cplx<R> q0 = ps[ym * L + x0];
     const cplx<R> q1 = ps[ym * L + x0 + 1];
-    cf.ey[0][0] = phase<R>(hp.sh2, q0.y);
-    cf.ey[1][0] = phase<R>(hp.sh2, q1.y);
+    cf.ey[0][0] = phase<R>(s2, q0.y);
+    cf.ey[1][0] = phase<R>(s2, q1.y);
+}
+
+// Two adjacent sites (x even) as one 16-byte (complex64) or two 16-byte
+// (complex128) accesses.
+template <typename R> __device__ __forceinline__ void ld_pair(const cplx<R>* p, cplx<R>& a, cplx<R>& b);
+template <> __device__ __forceinline__ void ld_pair<float>(const float2* p, float2& a, float2& b) {
+    const float4 q = *reinterpret_cast<const float4*>(p);
+    a = make_float2(q.x, q.y);
+    b = make_float2(q.z, q.w);
+}
+template <> __device__ __forceinline__ void ld_pair<double>(const double2* p, double2& a, double2& b) {
+    a = p[0];
+    b = p[1];
+}
+template <typename R> __device__ __forceinline__ void st_pair(cplx<R>* p, cplx<R> a, cplx<R> b);
+template <> __device__ __forceinline__ void st_pair<float>(float2* p, float2 a, float2 b) {
+    *reinterpret_cast<float4*>(p) = make_float4(a.x, a.y, b.x, b.y);
+}
+template <> __device__ __forceinline__ void st_pair<double>(double2* p, double2 a, double2 b) {
+    p[0] = a;
+    p[1] = b;
+}
+
+// Coefficients from a precomputed hop table hop[tau][y][x][mu] = th_g e^{i phi_mu}
+// (built once per field by hop_table_kernel): no sincos in the operator.
+template <typename R, int BY>
+__device__ __forceinline__ void load_coef_tab(Coef<R, BY>& cf, const cplx<R>* __restrict__ hop_s, const Geo& g) {
+    const int L = g.L;
+    const int x0 = 2 * g.a;
+    const int xm = x0 == 0 ? L - 1 : x0 - 1;
+#pragma unroll
+    for (int r = 0; r < BY; ++r) {
+        const int y = g.y0 + r;
+        cplx<R> h0x, h0y, h1x, h1y;
+        ld_pair<R>(hop_s + 2 * (y * L + x0), h0x, h0y);
+        ld_pair<R>(hop_s + 2 * (y * L + x0 + 1), h1x, h1y);
+        cf.ex[r][0] = hop_s[2 * (y * L + xm)];
+        cf.ex[r][1] = h0x;
+        cf.ex[r][2] = h1x;
+        cf.ey[0][r + 1] = h0y;
+        cf.ey[1][r + 1] = h1y;
+    }
+    const int ym = g.y0 == 0 ? L - 1 : g.y0 - 1;
+    cf.ey[0][0] = hop_s[2 * (ym * L + x0) + 1];
+    cf.ey[1][0] = hop_s[2 * (ym * L + x0 + 1) + 1];
 }
 
 // LDS exchange buffer: two alternating halves, each NV * (L/2)*L complex
 // (NV vectors exchanged together).  Callers size the dynamic LDS with
-// lds_bytes<R>(L, NV).
+// lds_bytes<R>(L, NV).  Within a vector's sub-buffer the edge values are kept
+// in four planes indexed by the patch column a, so a wave's writes and reads
+// are contiguous (bank-conflict free):
+//   x-groups: EL[row][a] (cx=0 site), ER[row][a] (cx=1 site), row = y >> 1
+//   y-groups: BOT[b][a] (r=0 site), TOP[b][a] (r=BY-1 site)
 __host__ __device__ constexpr int xstride(int L) { return (L >> 1) * L; }
 template <typename R>
 inline size_t lds_bytes(int L, int NV) { return (size_t)2 * NV * xstride(L) * sizeof(cplx<R>); }
@@ -108,126 +161,154 @@ inline size_t lds_bytes(int L, int NV) { return (size_t)2 * NV * xstride(L) * si
 template <typename R, int BY, int G, int NV>
 __device__ __forceinline__ void exchange(cplx<R> (*v)[2 * BY], cplx<R> (*pr)[BY], cplx<R>* lds, const Geo& g, int& phase) {
     const int L = g.L;
+    const int na = L >> 1;
     const int half = xstride(L);                 // per-vector sub-buffer
+    const int plane = half >> 1;                 // na * L/2
     cplx<R>* buf = lds + (phase & 1) * NV * half;
     if constexpr (G == 1 || G == 3) {
         constexpr int rc = (G == 1) ? 1 : 0;   // edge rows of group G have r % 2 == rc
-        const int x0 = 2 * g.a;
-        if (g.valid) {
 #pragma unroll
-            for (int n = 0; n < NV; ++n)
+        for (int n = 0; n < NV; ++n)
 #pragma unroll
-                for (int r = rc; r < BY; r += 2) {
-                    cplx<R>* row = buf + n * half + ((g.y0 + r) >> 1) * L;
-                    row[x0] = v[n][2 * r];
-                    row[x0 + 1] = v[n][2 * r + 1];
-                }
-        }
-        __syncthreads();
-        if (g.valid) {
-            const int xl = (x0 + L - 1) % L, xr = (x0 + 2) % L;
+            for (int r = rc; r < BY; r += 2) {
+                const int o = ((g.y0 + r) >> 1) * na + g.a;
+                buf[n * half + o] = v[n][2 * r];
+                buf[n * half + plane + o] = v[n][2 * r + 1];
+            }
+        lds_barrier();
+        const int am = g.a == 0 ? na - 1 : g.a - 1, ap = g.a + 1 == na ? 0 : g.a + 1;
 #pragma unroll
-            for (int n = 0; n < NV; ++n)
+        for (int n = 0; n < NV; ++n)
 #pragma unroll
-                for (int r = rc, j = 0; r < BY; r += 2, ++j) {
-                    const cplx<R>* row = buf + n * half + ((g.y0 + r) >> 1) * L;
-                    pr[n][2 * j] = row[xl];
-                    pr[n][2 * j + 1] = row[xr];
-                }
-        }
+            for (int r = rc, j = 0; r < BY; r += 2, ++j) {
+                const int rowo = ((g.y0 + r) >> 1) * na;
+                pr[n][2 * j] = buf[n * half + plane + rowo + am];
+                pr[n][2 * j + 1] = buf[n * half + rowo + ap];
+            }
     } else {
         constexpr int cc = (G == 2) ? 1 : 0;   // edge column of group G
-        const int na = L >> 1;
-        if (g.valid) {
+        const int yplane = g.nb * na;
 #pragma unroll
-            for (int n = 0; n < NV; ++n) {
-                cplx<R>* s = buf + n * half + 2 * (g.b * na + g.a);
-                s[0] = v[n][cc];
-                s[1] = v[n][2 * (BY - 1) + cc];
-            }
+        for (int n = 0; n < NV; ++n) {
+            const int o = g.b * na + g.a;
+            buf[n * half + o] = v[n][cc];
+            buf[n * half + yplane + o] = v[n][2 * (BY - 1) + cc];
         }
-        __syncthreads();
-        if (g.valid) {
-            const int bm = (g.b + g.nb - 1) % g.nb, bp = (g.b + 1) % g.nb;
+        lds_barrier();
+        const int bm = g.b == 0 ? g.nb - 1 : g.b - 1, bp = g.b + 1 == g.nb ? 0 : g.b + 1;
 #pragma unroll
-            for (int n = 0; n < NV; ++n) {
-                pr[n][0] = buf[n * half + 2 * (bm * na + g.a) + 1];
-                pr[n][1] = buf[n * half + 2 * (bp * na + g.a) + 0];
-            }
+        for (int n = 0; n < NV; ++n) {
+            pr[n][0] = buf[n * half + yplane + bm * na + g.a];
+            pr[n][1] = buf[n * half + bp * na + g.a];
         }
     }
     ++phase;
 }
 
-// v <- D_G v (SGN = +1) or D_G^{-1} v (SGN = -1; D^{-1} = cosh - sinh K since
-// K_g^2 = 1 on a perfect matching), given the edge partner values pr.
-template <typename R, int BY, int G, int SGN>
+// v <- D_G v given the edge partner values pr.  NORM: normalised factor
+// (I + th K_G), coefficients scaled by tanh.  Otherwise the plain factor
+// cosh + sinh K (SGN = +1) or its inverse cosh - sinh K (SGN = -1; K_g^2 = 1
+// on a perfect matching), coefficients scaled by sinh.
+template <typename R, int BY, int G, int SGN, bool NORM = false>
 __device__ __forceinline__ void factor_update(cplx<R> (&v)[2 * BY], const cplx<R> (&pr)[BY], const Coef<R, BY>& cf, const Hop<R>& hp) {
-    if constexpr (G == 1 || G == 3) {
+    static_assert(!NORM || SGN > 0, "normalised factors are applied forward only");
+    auto upd = [&](cplx<R> a, cplx<R> h, cplx<R> w) -> cplx<R> {
+        if constexpr (NORM) return pair_add<R>(a, h, w);
         const R c = (G == 1) ? hp.ch1 : hp.ch2;
+        const cplx<R> hh = SGN > 0 ? h : mk<R>(-h.x, -h.y);
+        return pair_upd<R>(c, a, hh, w);
+    };
+    auto updc = [&](cplx<R> a, cplx<R> h, cplx<R> w) -> cplx<R> {
+        if constexpr (NORM) return pair_addc<R>(a, h, w);
+        const R c = (G == 1) ? hp.ch1 : hp.ch2;
+        const cplx<R> hh = SGN > 0 ? h : mk<R>(-h.x, -h.y);
+        return pair_updc<R>(c, a, hh, w);
+    };
+    if constexpr (G == 1 || G == 3) {
         constexpr int rc = (G == 1) ? 1 : 0;
 #pragma unroll
         for (int r = 0, j = 0; r < BY; ++r) {
             if ((r & 1) != rc) {   // bond inside the patch, first site cx=0
-                const cplx<R> h = SGN > 0 ? cf.ex[r][1] : mk<R>(-cf.ex[r][1].x, -cf.ex[r][1].y);
                 const cplx<R> v0 = v[2 * r], v1 = v[2 * r + 1];
-                v[2 * r] = pair_upd<R>(c, v0, h, v1);
-                v[2 * r + 1] = pair_updc<R>(c, v1, h, v0);
+                v[2 * r] = upd(v0, cf.ex[r][1], v1);
+                v[2 * r + 1] = updc(v1, cf.ex[r][1], v0);
             } else {               // both sites on edge bonds
-                const cplx<R> hl = SGN > 0 ? cf.ex[r][0] : mk<R>(-cf.ex[r][0].x, -cf.ex[r][0].y);
-                const cplx<R> hr = SGN > 0 ? cf.ex[r][2] : mk<R>(-cf.ex[r][2].x, -cf.ex[r][2].y);
-                v[2 * r] = pair_updc<R>(c, v[2 * r], hl, pr[2 * j]);
-                v[2 * r + 1] = pair_upd<R>(c, v[2 * r + 1], hr, pr[2 * j + 1]);
+                v[2 * r] = updc(v[2 * r], cf.ex[r][0], pr[2 * j]);
+                v[2 * r + 1] = upd(v[2 * r + 1], cf.ex[r][2], pr[2 * j + 1]);
                 ++j;
             }
         }
     } else {
-        const R c = hp.ch2;
         constexpr int cc = (G == 2) ? 1 : 0;
         constexpr int ci = 1 - cc;
 #pragma unroll
         for (int r = 0; r < BY; r += 2) {   // inside column: pairs (r, r+1), r even
-            const cplx<R> h = SGN > 0 ? cf.ey[ci][r + 1] : mk<R>(-cf.ey[ci][r + 1].x, -cf.ey[ci][r + 1].y);
             const cplx<R> v0 = v[2 * r + ci], v1 = v[2 * (r + 1) + ci];
-            v[2 * r + ci] = pair_upd<R>(c, v0, h, v1);
-            v[2 * (r + 1) + ci] = pair_updc<R>(c, v1, h, v0);
+            v[2 * r + ci] = upd(v0, cf.ey[ci][r + 1], v1);
+            v[2 * (r + 1) + ci] = updc(v1, cf.ey[ci][r + 1], v0);
         }
 #pragma unroll
         for (int r = 1; r + 1 < BY; r += 2) {   // edge column: pairs (r, r+1), r odd
-            const cplx<R> h = SGN > 0 ? cf.ey[cc][r + 1] : mk<R>(-cf.ey[cc][r + 1].x, -cf.ey[cc][r + 1].y);
             const cplx<R> v0 = v[2 * r + cc], v1 = v[2 * (r + 1) + cc];
-            v[2 * r + cc] = pair_upd<R>(c, v0, h, v1);
-            v[2 * (r + 1) + cc] = pair_updc<R>(c, v1, h, v0);
+            v[2 * r + cc] = upd(v0, cf.ey[cc][r + 1], v1);
+            v[2 * (r + 1) + cc] = updc(v1, cf.ey[cc][r + 1], v0);
         }
-        const cplx<R> hb = SGN > 0 ? cf.ey[cc][0] : mk<R>(-cf.ey[cc][0].x, -cf.ey[cc][0].y);
-        const cplx<R> ht = SGN > 0 ? cf.ey[cc][BY] : mk<R>(-cf.ey[cc][BY].x, -cf.ey[cc][BY].y);
-        v[cc] = pair_updc<R>(c, v[cc], hb, pr[0]);
-        v[2 * (BY - 1) + cc] = pair_upd<R>(c, v[2 * (BY - 1) + cc], ht, pr[1]);
+        v[cc] = updc(v[cc], cf.ey[cc][0], pr[0]);
+        v[2 * (BY - 1) + cc] = upd(v[2 * (BY - 1) + cc], cf.ey[cc][BY], pr[1]);
     }
 }
 
-template <typename R, int BY, int G>
+template <typename R, int BY, int G, bool NORM = true>
 __device__ __forceinline__ void factor(cplx<R> (&v)[2 * BY], const Coef<R, BY>& cf, const Hop<R>& hp, cplx<R>* lds, const Geo& g,
                                        int& phase) {
     cplx<R> pr[1][BY];
     exchange<R, BY, G, 1>(reinterpret_cast<cplx<R>(*)[2 * BY]>(&v), pr, lds, g, phase);
-    factor_update<R, BY, G, 1>(v, pr[0], cf, hp);
+    factor_update<R, BY, G, 1, NORM>(v, pr[0], cf, hp);
 }
 
-// v <- B_tau v
+// Two independent factor applications sharing one exchange barrier:
+// v1 <- D_G(c1) v1 and v2 <- D_G(c2) v2 (different slices' coefficients).
+template <typename R, int BY, int G>
+__device__ __forceinline__ void factor2(cplx<R> (&v1)[2 * BY], const Coef<R, BY>& c1, cplx<R> (&v2)[2 * BY], const Coef<R, BY>& c2,
+                                        const Hop<R>& hp, cplx<R>* lds, const Geo& g, int& phase) {
+    cplx<R> vv[2][2 * BY];
+#pragma unroll
+    for (int k = 0; k < 2 * BY; ++k) { vv[0][k] = v1[k]; vv[1][k] = v2[k]; }
+    cplx<R> pr[2][BY];
+    exchange<R, BY, G, 2>(vv, pr, lds, g, phase);
+    factor_update<R, BY, G, 1, true>(v1, pr[0], c1, hp);
+    factor_update<R, BY, G, 1, true>(v2, pr[1], c2, hp);
+}
+
+// v1 <- B(c1) v1 / cn and v2 <- B(c2) v2 / cn (normalised factors) with 7
+// barriers instead of 14.
 template <typename R, int BY>
+__device__ __forceinline__ void apply_B2(cplx<R> (&v1)[2 * BY], const Coef<R, BY>& c1, cplx<R> (&v2)[2 * BY], const Coef<R, BY>& c2,
+                                         const Hop<R>& hp, cplx<R>* lds, const Geo& g, int& phase) {
+    factor2<R, BY, 4>(v1, c1, v2, c2, hp, lds, g, phase);
+    factor2<R, BY, 3>(v1, c1, v2, c2, hp, lds, g, phase);
+    factor2<R, BY, 2>(v1, c1, v2, c2, hp, lds, g, phase);
+    factor2<R, BY, 1>(v1, c1, v2, c2, hp, lds, g, phase);
+    factor2<R, BY, 2>(v1, c1, v2, c2, hp, lds, g, phase);
+    factor2<R, BY, 3>(v1, c1, v2, c2, hp, lds, g, phase);
+    factor2<R, BY, 4>(v1, c1, v2, c2, hp, lds, g, phase);
+}
+
+// v <- B_tau v (NORM = false) or B_tau v / cn (NORM = true, the default)
+template <typename R, int BY, bool NORM = true>
 __device__ __forceinline__ void apply_B(cplx<R> (&v)[2 * BY], const Coef<R, BY>& cf, const Hop<R>& hp, cplx<R>* lds, const Geo& g,
                                         int& phase) {
-    factor<R, BY, 4>(v, cf, hp, lds, g, phase);
-    factor<R, BY, 3>(v, cf, hp, lds, g, phase);
-    factor<R, BY, 2>(v, cf, hp, lds, g, phase);
-    factor<R, BY, 1>(v, cf, hp, lds, g, phase);
-    factor<R, BY, 2>(v, cf, hp, lds, g, phase);
-    factor<R, BY, 3>(v, cf, hp, lds, g, phase);
-    factor<R, BY, 4>(v, cf, hp, lds, g, phase);
+    factor<R, BY, 4, NORM>(v, cf, hp, lds, g, phase);
+    factor<R, BY, 3, NORM>(v, cf, hp, lds, g, phase);
+    factor<R, BY, 2, NORM>(v, cf, hp, lds, g, phase);
+    factor<R, BY, 1, NORM>(v, cf, hp, lds, g, phase);
+    factor<R, BY, 2, NORM>(v, cf, hp, lds, g, phase);
+    factor<R, BY, 3, NORM>(v, cf, hp, lds, g, phase);
+    factor<R, BY, 4, NORM>(v, cf, hp, lds, g, phase);
 }
 
-// v <- (first nfac factors of B_tau) v, nfac in [0, 7] (uniform over the block).
+// v <- (first nfac normalised factors of B_tau) v, nfac in [0, 7] (uniform
+// over the block); the caller multiplies by the product of their cosh.
 template <typename R, int BY>
 __device__ __forceinline__ void apply_B_partial(cplx<R> (&v)[2 * BY], int nfac, const Coef<R, BY>& cf, const Hop<R>& hp, cplx<R>* lds,
                                                 const Geo& g, int& phase) {
@@ -245,21 +326,44 @@ __device__ __forceinline__ void apply_B_partial(cplx<R> (&v)[2 * BY], int nfac, 
 // 2 x 16 B (c128) access per row, adjacent threads adjacent in memory.
 template <typename R, int BY>
 __device__ __forceinline__ void load_patch(cplx<R> (&v)[2 * BY], const cplx<R>* __restrict__ s, const Geo& g) {
+    const cplx<R>* p = s + g.y0 * g.L + 2 * g.a;
 #pragma unroll
-    for (int r = 0; r < BY; ++r) {
-        const cplx<R>* p = s + (g.y0 + r) * g.L + 2 * g.a;
-        v[2 * r] = p[0];
-        v[2 * r + 1] = p[1];
-    }
+    for (int r = 0; r < BY; ++r) ld_pair<R>(p + r * g.L, v[2 * r], v[2 * r + 1]);
 }
 
 template <typename R, int BY>
 __device__ __forceinline__ void store_patch(cplx<R>* __restrict__ s, const cplx<R> (&v)[2 * BY], const Geo& g) {
+    cplx<R>* p = s + g.y0 * g.L + 2 * g.a;
 #pragma unroll
-    for (int r = 0; r < BY; ++r) {
-        cplx<R>* p = s + (g.y0 + r) * g.L + 2 * g.a;
-        p[0] = v[2 * r];
-        p[1] = v[2 * r + 1];
+    for (int r = 0; r < BY; ++r) st_pair<R>(p + r * g.L, v[2 * r], v[2 * r + 1]);
+}
+
+// Hop table of one field: hop[tau][y][x][mu] = th_g(x,y,mu) e^{i phi_mu(x,y)},
+// th = tanh(t dtau) for the x-bonds of even sites (group 1), tanh(t dtau/2)
+// otherwise.  Accurate sincos: built once per field, read by every iteration.
+template <typename R>
+__global__ void hop_table_kernel(const R* __restrict__ phi, long long phi_cs, cplx<R>* __restrict__ hop, int nchain, int L, int Lt,
+                                 double th1, double th2) {
+    const int Vs = L * L;
+    const long long n = (long long)nchain * Lt * Vs;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int s = (int)(i % Vs);
+        const long long q = i / Vs;            // chain * Lt + tau
+        const int chain = (int)(q / Lt), tau = (int)(q % Lt);
+        const int x = s % L, y = s / L;
+        const R* ph = phi + chain * phi_cs + ((long long)tau * Vs + s) * 2;
+        cplx<R>* h = hop + (q * Vs + s) * 2;
+        R sx, cx, sy, cy;
+        if constexpr (sizeof(R) == 4) {
+            sincosf(ph[0], &sx, &cx);
+            sincosf(ph[1], &sy, &cy);
+        } else {
+            sincos(ph[0], &sx, &cx);
+            sincos(ph[1], &sy, &cy);
+        }
+        const R tx = ((x + y) & 1) ? (R)th2 : (R)th1;
+        h[0] = mk<R>(tx * cx, tx * sx);
+        h[1] = mk<R>((R)th2 * cy, (R)th2 * sy);
     }
 }
 

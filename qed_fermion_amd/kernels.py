@@ -64,6 +64,36 @@ def apply(op, phi, x, L, Ltau, dtau=0.1, t=1.0, out=None):
     return y.view(x.shape)
 
 
+def hop_table(phi, L, Ltau, dtau=0.1, t=1.0, out=None):
+    """Per-link factor coefficients tanh(a_g) e^{i phi} of each field in phi [nchain, 2N]."""
+    N = L * L * Ltau
+    _check_dev(phi)
+    nchain = phi.numel() // (2 * N)
+    cd = torch.complex64 if phi.dtype == torch.float32 else torch.complex128
+    hop = torch.empty(nchain, 2 * N, dtype=cd, device=phi.device) if out is None else out
+    call("qf_hop_table", _prec(phi), ptr(phi), 2 * N, nchain, L, Ltau, float(dtau), float(t), ptr(hop), stream_handle())
+    return hop
+
+
+def apply_hop(op, hop, x, L, Ltau, dtau=0.1, t=1.0, out=None):
+    """y = op x with the operator given by its hop table (hop [nchain, 2N] complex; nchain 1 = shared)."""
+    code = {"M": _lib.QF_OP_M, "Mdag": _lib.QF_OP_MDAG, "MdagM": _lib.QF_OP_MDAGM, "B": _lib.QF_OP_B}[op]
+    N = L * L * Ltau
+    x2 = x.reshape(-1, N)
+    _check_dev(hop, x2)
+    if hop.dtype != x.dtype:
+        raise QfError(_lib.QF_ERR_INVALID, f"hop table dtype {hop.dtype} does not match vector dtype {x.dtype}")
+    nrhs = x2.shape[0]
+    nchain = hop.numel() // (2 * N)
+    hop_cs = 0 if nchain == 1 else 2 * N
+    if nchain not in (1, nrhs):
+        raise QfError(_lib.QF_ERR_INVALID, f"hop table holds {nchain} fields for {nrhs} right-hand sides")
+    y = torch.empty_like(x2) if out is None else out.reshape(-1, N)
+    call("qf_apply_hop", code, _prec(x), ptr(hop), hop_cs, ptr(x2), ptr(y), nrhs, L, Ltau, float(dtau), float(t),
+         stream_handle())
+    return y.view(x.shape)
+
+
 def apply_partial_B(phi, x, L, Ltau, dtau, nfac, t=1.0):
     """y_tau = Q_nfac x_tau: the first nfac factors of B_tau = D4 D3 D2 D1 D2 D3 D4 applied per slice."""
     if not 0 <= nfac <= 7:

@@ -42,7 +42,7 @@ def test_trajectory_matches_oracle(dev, precision, tol):
         assert relerr(eng.p[i].cpu().numpy(), p1) < tol
         assert abs(eng.H0[i].item() - H0) < tol * abs(H0)
         assert abs((eng.H1[i] - eng.H0[i]).item() - (H1 - H0)) < 1e-6 + tol * abs(H0)
-    assert eng.err.max().item() < 1e-11
+    assert eng.err.max().item() < (1e-11 if precision == "fp64" else 2e-12)
 
 
 def test_graph_replay_matches_eager(dev):
