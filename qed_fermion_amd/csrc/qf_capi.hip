@@ -126,7 +126,7 @@ double cosh_prefix(int nfac, double dtau, double t) {
 
 int check_lattice(int L, int Lt, int nrhs) {
     if (L < 2 || (L & 1)) return fail(QF_ERR_ODD_L, "L=%d: the checkerboard bond groups need an even L >= 2", L);
-    if (Lt < 2) return fail(QF_ERR_INVALID, "Ltau=%d must be >= 2", Lt);
+    if (Lt < 1) return fail(QF_ERR_INVALID, "Ltau=%d must be >= 1", Lt);
     if (nrhs < 1 || nrhs > 65535) return fail(QF_ERR_INVALID, "nrhs=%d out of range [1, 65535]", nrhs);
     if (pick_by(L, 0) == 0) return fail(QF_ERR_UNSUPPORTED, "L=%d: no patch shape fits a 512/1024-thread work-group", L);
     return QF_OK;

@@ -1,0 +1,86 @@
+"""StochaticEstimator: the multi-RHS solve path of qed_fermion/stochastic_estimator.py.
+
+Same class and method names (the reference's spelling), same tensor shapes:
+
+  random_vec_bin()          -> eta [Nrv, Ltau*Ly*Lx], entries (+-1 +- i)/sqrt(2)      (210-224)
+  set_eta_G_eta(boson, eta) -> self.G_eta = (M^dag M)^{-1} M^dag eta = M^{-1} eta      (409-433)
+
+MI355X difference: the reference repeats the boson field Nrv times
+(``boson.repeat(Nrv, 1)``, 429) so its batched kernels see one field per RHS;
+here all Nrv right-hand sides share ONE field (phi stride 0), so M^dag and
+every PCG iteration read phi / its hop table once per launch for the whole
+batch, in one multi-RHS launch per iteration.  The FFT observables built on
+G_eta (T1..Tvv, 1393-1651) are outside this package.
+"""
+import math
+import os
+
+import torch
+
+from . import kernels
+from .fermion_obsr_graph_runner import GEtaGraphRunner
+
+
+class StochaticEstimator:
+    def __init__(self, hmc, cuda_graph_se=False, Nrv=None, max_iter_se=None):
+        self.hmc_sampler = hmc
+        self.Nrv = int(Nrv if Nrv is not None else os.getenv("Nrv", "20"))
+        self.max_iter_se = int(max_iter_se if max_iter_se is not None else os.getenv("max_iter_se", "100"))
+        self.Lx, self.Ly, self.Ltau = hmc.Lx, hmc.Ly, hmc.Ltau
+        self.dtau = hmc.dtau
+        self.Vs = hmc.Vs
+        self.cuda_graph_se = cuda_graph_se
+        self.device = hmc.device
+        self.dtype = hmc.dtype
+        self.cdtype = hmc.cdtype
+        self.graph_memory_pool = hmc.graph_memory_pool
+        self.G_eta_graph_runner = GEtaGraphRunner(self)
+        self.num_samples = lambda nrv: math.comb(nrv, 2)
+        self.batch_size = lambda nrv: int(nrv * 0.1)
+        self.G_eta = None
+        self.eta = None
+        if hmc.precon_table is None and hmc.dtau <= 0.1 and hmc.precon_on:
+            hmc.reset_precon()
+
+    def initialize(self):
+        self.indices_r2 = torch.combinations(torch.arange(self.Nrv, device=self.device), r=2, with_replacement=False)
+
+    def init_cuda_graph(self):
+        if self.cuda_graph_se:
+            dummy_eta = torch.zeros((self.Nrv, self.Ltau * self.Vs), device=self.device, dtype=self.cdtype)
+            dummy_boson = torch.zeros(1, 2, self.Lx, self.Ly, self.Ltau, device=self.device, dtype=self.dtype)
+            self.graph_memory_pool = self.G_eta_graph_runner.capture(dummy_boson, dummy_eta, self.graph_memory_pool)
+
+    def random_vec_bin(self):
+        """Nrv Z2 x Z2 noise vectors (+-1 +- i)/sqrt(2), [Nrv, Lx*Ly*Ltau] (stochastic_estimator.py:210-224)."""
+        n = self.Lx * self.Ly * self.Ltau
+        re = torch.randint(0, 2, (self.Nrv, n), device=self.device).to(torch.float64) * 2 - 1
+        im = torch.randint(0, 2, (self.Nrv, n), device=self.device).to(torch.float64) * 2 - 1
+        return torch.complex(re, im).div_(math.sqrt(2)).to(self.cdtype)
+
+    def set_eta_G_eta(self, boson, eta):
+        """G_eta = M^{-1} eta for all Nrv noise vectors (stochastic_estimator.py:409-425)."""
+        self.eta = eta
+        if self.cuda_graph_se and self.G_eta_graph_runner.graph is not None:
+            G_eta = self.G_eta_graph_runner(boson, eta)
+        else:
+            G_eta = self.set_eta_G_eta_inner(boson, eta)
+        self.G_eta = G_eta
+
+    def set_eta_G_eta_inner(self, boson, eta):
+        """psi = M^dag eta; G_eta = (M^dag M)^{-1} psi with one shared field (stochastic_estimator.py:427-433)."""
+        hmc = self.hmc_sampler
+        phi = boson.permute([0, 4, 3, 2, 1]).reshape(1, -1)
+        real = torch.float32 if eta.dtype == torch.complex64 else torch.float64
+        phi = phi.to(real).contiguous()
+        eta = eta.contiguous()
+        psi = kernels.apply("Mdag", phi, eta, self.Lx, self.Ltau, self.dtau)
+        ptab = hmc._ptab()
+        if hmc.precision == "mixed":
+            G_eta, _, _ = kernels.pcg_mixed_solve(phi.double(), psi.to(torch.complex128), self.Lx, self.Ltau, self.dtau,
+                                                  ptab=ptab, rtol=hmc.cg_rtol, max_iter=self.max_iter_se,
+                                                  ru_period=hmc.ru_period)
+        else:
+            G_eta, _, _ = kernels.pcg_solve(phi, psi, self.Lx, self.Ltau, self.dtau, ptab=ptab, rtol=hmc.cg_rtol,
+                                            max_iter=self.max_iter_se)
+        return G_eta

@@ -1,0 +1,230 @@
+"""The drop-in Python API (reference-shaped classes and the _C shim) on the GPU,
+against the oracle and the dense brute force."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import dense_ref as D
+from helpers import make_phi, make_vec, relerr
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _sampler(L=4, Lt=30, bs=2, precision="fp64", **kw):
+    from qed_fermion_amd.hmc_sampler import HmcSampler
+    kw.setdefault("cuda_graph", False)
+    return HmcSampler(Lx=L, Ltau=Lt, J=1.0, bs=bs, precision=precision, device="cuda", **kw)
+
+
+def _ref_layout(phi_np):
+    """oracle layout [bs, Lt, L, L, 2] -> reference boson [bs, 2, Lx, Ly, Ltau] (torch, cuda)."""
+    return torch.from_numpy(phi_np).permute(0, 4, 3, 2, 1).contiguous().cuda()
+
+
+def test_C_shim_matches_oracle(dev):
+    from qed_fermion_amd import _C, precon
+    L, Lt, bs = 6, 30, 2
+    phi = make_phi(L, Lt, "uniform", seed=1, nchain=bs)
+    x = make_vec(L, Lt, seed=2, nrhs=bs)
+    boson = torch.from_numpy(phi.reshape(bs, -1)).float().cuda()
+    xc = torch.from_numpy(x).to(torch.complex64).cuda()
+    y = _C.mhm_vec(boson, xc, L, 0.1, 4, 8).cpu().numpy()
+    z = _C.mh_vec(boson, xc, L, 0.1, 4, 8).cpu().numpy()
+    for i in range(bs):
+        assert relerr(y[i], O.apply_MdagM(phi[i], x[i], L, Lt)) < 2e-5
+        assert relerr(z[i], O.apply_Mdag(phi[i], x[i], L, Lt)) < 2e-5
+    csr = precon.coo_to_csr(precon.get_precon_man(L, Lt), device="cuda")
+    w = _C.precon_vec(xc, csr, L).cpu().numpy()
+    tab = O.precon_table(Lt)
+    for i in range(bs):
+        assert relerr(w[i], O.apply_P(tab, x[i], L * L, Lt)) < 1e-6
+    # b_vec_per_tau on one slice: full B, and the intermediates in force_f_fast's order
+    t = 3
+    bt = boson[0].view(Lt, -1)[t]
+    v = xc[0].view(Lt, -1)[t]
+    full = _C.b_vec_per_tau(bt, v, L, 0.1, False).cpu().numpy()
+    assert relerr(full, O.apply_B(phi[0, t], x[0].reshape(Lt, -1)[t], L)) < 2e-5
+    inter = _C.b_vec_per_tau(bt, v, L, 0.1, True).cpu().numpy()
+    for k, m in enumerate((5, 4, 3, 2, 1, 6)):
+        assert relerr(inter[k], O.apply_B(phi[0, t], x[0].reshape(Lt, -1)[t], L, nfac=m)) < 2e-5
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 1e-7), ("fp32", 2e-4)])
+def test_force_f_fast_matches_oracle(dev, precision, tol):
+    L, Lt, bs = 4, 30, 2
+    hmc = _sampler(L, Lt, bs, precision, max_iter=800)
+    hmc.cg_rtol = 1e-12 if precision != "fp32" else 1e-6
+    phi = make_phi(L, Lt, "gauss", seed=5, nchain=bs)
+    psi = make_vec(L, Lt, seed=6, nrhs=bs)
+    boson = _ref_layout(phi).to(hmc.dtype)
+    Ft, xi, iters, err = hmc.force_f_fast(torch.from_numpy(psi).to(hmc.cdtype).cuda(), boson, None)
+    assert Ft.shape == (bs, 2, L, L, Lt) and xi.shape == (bs, L * L * Lt)
+    tab = O.precon_table(Lt)
+    for i in range(bs):
+        xr, _, _ = O.pcg(phi[i], psi[i], L, Lt, table=tab, rtol=1e-13, max_iter=4000)
+        Fr = O.force(phi[i], xr, L, Lt)                                      # [Lt, L, L, 2]
+        Fr_ref_layout = np.transpose(Fr, (3, 2, 1, 0))                      # [2, Lx, Ly, Lt]
+        assert relerr(Ft[i].cpu().numpy(), Fr_ref_layout) < tol
+        assert relerr(xi[i].cpu().numpy(), xr) < tol
+
+
+def test_boson_api_matches_oracle(dev):
+    L, Lt, bs = 4, 10, 2
+    hmc = _sampler(L, Lt, bs, "fp64")
+    cJ, cK = hmc.couplings()
+    phi = make_phi(L, Lt, "gauss", seed=8, nchain=bs)
+    boson = _ref_layout(phi)
+    for compact in (False, True):
+        hmc.compact = compact
+        S_tau = hmc.action_boson_tau(boson).cpu().numpy()
+        S_pl = hmc.action_boson_plaq_matfree(boson).cpu().numpy()
+        Fp = hmc.force_b_plaq_matfree(boson).cpu().numpy()
+        Ftau = (hmc.force_b_tau_cmp if compact else hmc.force_b_tau_noncmp)(boson).cpu().numpy()
+        for i in range(bs):
+            assert abs(S_tau[i] - O.action_tau(phi[i], L, Lt, cJ, compact)) < 1e-10 * abs(S_tau[i]) + 1e-12
+            assert abs(S_pl[i] - O.action_plaq(phi[i], L, Lt, cK)) < 1e-10 * abs(S_pl[i]) + 1e-12
+            assert relerr(Fp[i], np.transpose(O.force_b_plaq(phi[i], L, Lt, cK), (3, 2, 1, 0))) < 1e-12
+            assert relerr(Ftau[i], np.transpose(O.force_b_tau(phi[i], L, Lt, cJ, compact), (3, 2, 1, 0))) < 1e-12
+    curl = hmc.curl_phi(boson)
+    assert curl.shape == (bs, L, L, Lt)
+
+
+def test_force_graph_runner_replays_force_f_fast(dev):
+    from qed_fermion_amd.force_graph_runner import ForceGraphRunner
+    L, Lt, bs = 6, 30, 2
+    hmc = _sampler(L, Lt, bs, "fp32", max_iter=100)
+    phi = make_phi(L, Lt, "gauss", seed=9, nchain=bs)
+    boson = _ref_layout(phi).float()
+    psi = torch.from_numpy(make_vec(L, Lt, seed=10, nrhs=bs)).to(torch.complex64).cuda()
+    Ft0, xi0, _, err0 = hmc.force_f_fast(psi, boson, None)
+    runner = ForceGraphRunner(hmc)
+    runner.capture(torch.zeros_like(psi), torch.zeros_like(boson), 100)
+    for _ in range(2):
+        Ft, xi, err = runner(psi, boson)
+        torch.cuda.synchronize()
+        assert torch.equal(Ft, Ft0) and torch.equal(xi, xi0) and torch.equal(err, err0)
+
+
+def test_leapfrog_cmp_and_runner(dev):
+    from qed_fermion_amd.metropolis_graph_runner import LeapfrogCmpGraphRunner
+    L, Lt, bs = 4, 10, 2
+    hmc = _sampler(L, Lt, bs, "fp64")
+    cJ, cK = hmc.couplings()
+    phi = make_phi(L, Lt, "gauss", seed=11, nchain=bs)
+    p = np.random.default_rng(12).standard_normal(phi.shape)
+    x, pt = _ref_layout(phi), _ref_layout(p)
+    dt = torch.tensor([0.05, 0.02], dtype=torch.float64, device="cuda").view(-1, 1, 1, 1, 1)
+    mask = torch.ones(1, 1, 1, 1, Lt, dtype=torch.float64, device="cuda")
+    Fp, Ft = hmc.force_b_plaq_matfree(x), hmc.force_b_tau_noncmp(x)
+    x1, p1, Fp1, Ft1 = hmc.leapfrog_cmp(x, pt, dt, mask, Fp, Ft)
+    for i in range(bs):
+        xs, ps = phi[i].copy(), p[i].copy()
+        h = [0.05, 0.02][i]
+        fp, ft = O.force_b_plaq(xs, L, Lt, cK), O.force_b_tau(xs, L, Lt, cJ)
+        for _ in range(5):
+            ps = ps + (fp + ft) * h / 10
+            xs = xs + ps * h / 5
+            fp, ft = O.force_b_plaq(xs, L, Lt, cK), O.force_b_tau(xs, L, Lt, cJ)
+            ps = ps + (fp + ft) * h / 10
+        assert relerr(x1[i].cpu().numpy(), np.transpose(xs, (3, 2, 1, 0))) < 1e-13
+        assert relerr(p1[i].cpu().numpy(), np.transpose(ps, (3, 2, 1, 0))) < 1e-13
+    hmc.cuda_graph = True
+    runner = LeapfrogCmpGraphRunner(hmc)
+    runner.capture(torch.zeros_like(x), torch.zeros_like(x), torch.zeros_like(dt), mask.clone(), torch.zeros_like(x),
+                   torch.zeros_like(x))
+    out = runner(x, pt, dt, mask, Fp, Ft)
+    torch.cuda.synchronize()
+    for a, b in zip(out, (x1, p1, Fp1, Ft1)):
+        assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("cuda_graph", [False, True])
+def test_metropolis_update_runs(dev, cuda_graph):
+    L, Lt, bs = 6, 30, 3
+    hmc = _sampler(L, Lt, bs, "mixed", max_iter=300, cuda_graph=cuda_graph)
+    hmc.delta_t_tensor.fill_(0.02)
+    for _ in range(2):
+        before = hmc.boson.clone()
+        boson, accp, iters, r_err = hmc.metropolis_update()
+        torch.cuda.synchronize()
+        assert boson.shape == (bs, 2, L, L, Lt)
+        assert torch.equal(boson[~accp], before[~accp])
+        assert torch.isfinite(r_err).all()
+    assert all(len(q) == 2 for q in hmc.threshold_queue)
+    assert all(0.0 <= v <= 1.0 for q in hmc.threshold_queue for v in q)
+
+
+def test_stochastic_estimator_solve(dev):
+    from qed_fermion_amd.stochastic_estimator import StochaticEstimator
+    L, Lt = 4, 30
+    hmc = _sampler(L, Lt, 1, "fp64")
+    hmc.cg_rtol = 1e-12
+    se = StochaticEstimator(hmc, cuda_graph_se=False, Nrv=8, max_iter_se=1000)
+    se.initialize()
+    eta = se.random_vec_bin()
+    e = eta.cpu().numpy()
+    assert np.allclose(np.abs(e.real), 1 / math.sqrt(2)) and np.allclose(np.abs(e.imag), 1 / math.sqrt(2))
+    phi = make_phi(L, Lt, "gauss", seed=13)
+    boson = _ref_layout(phi)
+    se.set_eta_G_eta(boson, eta)
+    M = D.M_mat(L, Lt, phi[0])
+    G_ref = np.linalg.solve(M, e.T).T                      # G eta = M^{-1} eta
+    assert relerr(se.G_eta.cpu().numpy(), G_ref) < 1e-9
+    se.cuda_graph_se = True
+    se.init_cuda_graph()
+    se.set_eta_G_eta(boson, eta)
+    assert relerr(se.G_eta.cpu().numpy(), G_ref) < 1e-9
+
+
+def test_z2_noise_orthogonality(dev):
+    """E[eta eta^dag] = I for the (+-1 +- i)/sqrt(2) noise (stochastic_estimator.py:264-289)."""
+    from qed_fermion_amd.stochastic_estimator import StochaticEstimator
+    hmc = _sampler(4, 30, 1, "fp64")
+    se = StochaticEstimator(hmc, Nrv=4000)
+    eta = se.random_vec_bin()[:, :16].to(torch.complex128)
+    C = (eta.T @ eta.conj()) / eta.shape[0]
+    assert torch.allclose(torch.diagonal(C).real, torch.ones(16, dtype=torch.float64, device="cuda"), atol=1e-12)
+    off = C - torch.diag(torch.diagonal(C))
+    assert off.abs().max().item() < 0.1
+
+
+def test_cg_benchmark_preconditioner_helps(dev):
+    from qed_fermion_amd.cg_hmc_sampler import CgHmcSampler
+    cg = CgHmcSampler(J=1.0, Lx=4, Ltau=40, bs=1, precision="fp32")
+    out = cg.benchmark(bs=3)
+    assert len(out) == 9 and out[0] > 0
+    r = cg.last_benchmark
+    pi_idx = 3                                              # fields: 3 uniform, pi flux, 2 noisy
+    assert r["iters_precon"][pi_idx] < r["iters_plain"][pi_idx]
+
+
+def test_autograd_force(dev):
+    """FermionAction.backward is -Ft; checked against the oracle force and central differences."""
+    from qed_fermion_amd.autograd import fermion_action
+    from qed_fermion_amd import kernels
+    L, Lt = 4, 30
+    phi = make_phi(L, Lt, "gauss", seed=14)
+    psi = make_vec(L, Lt, seed=15)
+    ptab = kernels.precon_table(Lt, device="cuda")
+    ph = torch.from_numpy(phi.reshape(1, -1)).cuda().requires_grad_(True)
+    ps = torch.from_numpy(psi).cuda()
+    S = fermion_action(ph, ps, L, Lt, ptab=ptab, rtol=1e-14, max_iter=3000)
+    S.sum().backward()
+    xr, _, _ = O.pcg(phi[0], psi[0], L, Lt, table=O.precon_table(Lt), rtol=1e-14, max_iter=4000)
+    assert relerr(ph.grad[0].cpu().numpy(), -O.force(phi[0], xr, L, Lt).ravel()) < 1e-9
+    eps = 1e-5
+    for k in np.random.default_rng(16).choice(phi.size, 6, replace=False):
+        d = torch.zeros_like(ph)
+        d.view(-1)[k] = eps
+        with torch.no_grad():
+            Sp = fermion_action(ph + d, ps, L, Lt, ptab=ptab, rtol=1e-14, max_iter=3000).item()
+            Sm = fermion_action(ph - d, ps, L, Lt, ptab=ptab, rtol=1e-14, max_iter=3000).item()
+        fd = (Sp - Sm) / (2 * eps)
+        g = ph.grad.view(-1)[k].item()
+        assert abs(fd - g) < 1e-5 * max(1.0, abs(g)), (k, fd, g)
+    ph2 = ph.detach().clone().requires_grad_(True)
+    fermion_action(ph2, ps, L, Lt, ptab=ptab, rtol=1e-14, max_iter=3000, recompute=False).sum().backward()
+    assert torch.equal(ph2.grad, ph.grad)

@@ -1,0 +1,103 @@
+"""Parity at the benchmark configurations' full sizes (L=24..66), where the
+oracle is only run for single operator / force applications, plus
+size-independent properties: the true residual of the returned solutions,
+mixed vs FP64 agreement, and central differences of the bilinear form
+Re(a^dag M(phi) xi) that the adjoint force kernel differentiates."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import make_phi, make_vec, relerr
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _k():
+    from qed_fermion_amd import kernels
+    return kernels
+
+
+def _t(a, dtype):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device="cuda", dtype=dtype)
+
+
+@pytest.mark.parametrize("L,Lt", [(36, 360), (66, 660)])
+def test_operator_full_size(dev, L, Lt):
+    k = _k()
+    print(f"[full operator] L={L} Lt={Lt}", flush=True)
+    phi = make_phi(L, Lt, "uniform", seed=L)
+    x = make_vec(L, Lt, seed=L + 1)
+    ref = O.apply_MdagM(phi[0], x[0], L, Lt)
+    y64 = k.apply("MdagM", _t(phi.reshape(1, -1), torch.float64), _t(x, torch.complex128), L, Lt).cpu().numpy()
+    assert relerr(y64[0], ref) < 1e-12
+    y32 = k.apply("MdagM", _t(phi.reshape(1, -1), torch.float32), _t(x, torch.complex64), L, Lt).cpu().numpy()
+    assert relerr(y32[0], ref) < 2e-5
+    hop = k.hop_table(_t(phi.reshape(1, -1), torch.float32), L, Lt)
+    yh = k.apply_hop("MdagM", hop, _t(x, torch.complex64), L, Lt).cpu().numpy()
+    assert relerr(yh[0], ref) < 2e-5
+
+
+def test_shared_field_multi_rhs_full_size(dev):
+    """The batching configuration: L=48, Ltau=480, many right-hand sides sharing one field."""
+    k = _k()
+    L, Lt, nrhs = 48, 480, 16
+    phi = make_phi(L, Lt, "uniform", seed=3)
+    rng = np.random.default_rng(4)
+    N = L * L * Lt
+    eta = ((rng.integers(0, 2, (nrhs, N)) * 2 - 1) + 1j * (rng.integers(0, 2, (nrhs, N)) * 2 - 1)) / np.sqrt(2)
+    hop = k.hop_table(_t(phi.reshape(1, -1), torch.float32), L, Lt)
+    y = k.apply_hop("MdagM", hop, _t(eta, torch.complex64), L, Lt).cpu().numpy()
+    for i in (0, nrhs - 1):
+        assert relerr(y[i], O.apply_MdagM(phi[0], eta[i], L, Lt)) < 2e-5
+
+
+def test_force_full_size_c5(dev):
+    """Adjoint force at L=66, Ltau=660 (2,874,960 sites) against the oracle, and against central
+    differences (eps=1e-5, 64 random bonds) of F_k = 2 d/dphi_k Re(a^dag M(phi) xi), a = M(phi0) xi."""
+    k = _k()
+    L, Lt = 66, 660
+    print("[force C5]", flush=True)
+    phi = make_phi(L, Lt, "uniform", seed=66)
+    xi = make_vec(L, Lt, seed=67)
+    phid = _t(phi.reshape(1, -1), torch.float64)
+    xid = _t(xi, torch.complex128)
+    F = k.fermion_force(phid, xid, L, Lt)
+    Fr = O.force(phi[0], xi[0], L, Lt).ravel()
+    assert relerr(F[0].cpu().numpy(), Fr) < 1e-12
+    a = k.apply("M", phid, xid, L, Lt)
+    eps = 1e-5
+    for kk in np.random.default_rng(68).choice(phi.size, 64, replace=False):
+        pp, pm = phid.clone(), phid.clone()
+        pp.view(-1)[kk] += eps
+        pm.view(-1)[kk] -= eps
+        dy = k.apply("M", pp, xid, L, Lt) - k.apply("M", pm, xid, L, Lt)   # zero away from the bond's slice
+        fd = 2.0 * torch.real(torch.vdot(a.view(-1), dy.view(-1))).item() / (2 * eps)
+        assert abs(fd - Fr[kk]) < 1e-6 * max(1.0, abs(Fr[kk])), (kk, fd, Fr[kk])
+
+
+def test_mixed_vs_fp64_c2(dev):
+    """Plain-method configuration (L=24, Ltau=240, phi ~ N(0, 0.3^2)): FP64 and mixed solves to
+    1e-10; the oracle recomputes both true residuals; mixed within 1e-7 of FP64, and the force
+    and S_f from both agree to the same level."""
+    k = _k()
+    L, Lt, nrhs = 24, 240, 2
+    print("[mixed C2]", flush=True)
+    phi = make_phi(L, Lt, "gauss", seed=24, nchain=nrhs)
+    R = make_vec(L, Lt, seed=25, nrhs=nrhs)
+    phid = _t(phi.reshape(nrhs, -1), torch.float64)
+    b = k.apply("Mdag", phid, _t(R, torch.complex128), L, Lt)
+    ptab = k.precon_table(Lt, device="cuda")
+    x64, it64, e64 = k.pcg_solve(phid, b, L, Lt, ptab=ptab, rtol=1e-10, max_iter=3000)
+    xm, itm, em = k.pcg_mixed_solve(phid, b, L, Lt, ptab=ptab, rtol=1e-10, max_iter=3000)
+    bn, x64n, xmn = b.cpu().numpy(), x64.cpu().numpy(), xm.cpu().numpy()
+    F64 = k.fermion_force(phid, x64, L, Lt)
+    Fm = k.fermion_force(phid, xm, L, Lt)
+    for i in range(nrhs):
+        assert relerr(O.apply_MdagM(phi[i], x64n[i], L, Lt), bn[i]) <= 1.01e-10
+        assert relerr(O.apply_MdagM(phi[i], xmn[i], L, Lt), bn[i]) <= 1.01e-10
+        assert relerr(xmn[i], x64n[i]) < 1e-7
+        assert relerr(Fm[i].cpu().numpy(), F64[i].cpu().numpy()) < 1e-7
+        s64 = np.vdot(bn[i], x64n[i]).real
+        sm = np.vdot(bn[i], xmn[i]).real
+        assert abs(sm - s64) < 1e-7 * abs(s64)
