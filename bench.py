@@ -56,6 +56,7 @@ def parse():
     ap.add_argument("--roof-reps", type=int, default=20)
     ap.add_argument("--cpu-iters", type=int, default=0, help="oracle PCG iterations for cpu_baseline (0 = auto, ~15 s)")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="use the built-in tilings instead of the autotuner")
     ap.add_argument("--seed", type=int, default=2024)
     return ap.parse_args()
 
@@ -95,6 +96,23 @@ def mdagm_roofline(torch, K, L, Lt, nrhs, reps, dev, use_hop=True):
             "nrhs": nrhs, "ms_per_launch": ms, "bytes_per_launch": algo, "GBps": algo / (ms * 1e-3) / 1e9}
 
 
+def traffic_of(L, Lt, nrhs):
+    """HBM bytes per launch of the roofline kernel at this shape, as measured by
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by a copy-kernel
+    calibration) and recorded in profiles/mdagm_traffic.json; None if that
+    shape was not measured."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "mdagm_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for e in rec.get("entries", []):
+        if (e.get("L"), e.get("Ltau"), e.get("nrhs")) == (L, Lt, nrhs):
+            return e.get("traffic_bytes")
+    return None
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -109,6 +127,16 @@ def main():
     dev = torch.device("cuda:0")
     L, Lt, bs = args.L, args.Ltau, args.bs
     N = L * L * Lt
+
+    # ---------------------------------------------------------- tilings (autotuner)
+    tilings = {}
+    if not args.no_tune:
+        from qed_fermion_amd import autotune
+        t0 = time.time()
+        for prec, nrhs, kind in ((0, bs, "pcg"), (1, 1, "pcg"), (1, bs, "pcg"), (0, args.roof_nrhs, "operator")):
+            hit = autotune.tune(prec, L, Lt, nrhs, kind=kind, persist=False)
+            tilings[f"{kind}:{'c64' if prec == 0 else 'c128'}:nrhs={nrhs}"] = hit
+        log(f"[bench] autotuned in {time.time() - t0:.1f}s: {tilings}")
     rng = np.random.default_rng(args.seed)
     phi0 = torch.from_numpy(rng.uniform(-np.pi, np.pi, (bs, 2 * N))).to(dev)
     common = dict(dtau=0.1, J=1.0, K=1.0, compact=False, m=1.0, delta_t=0.06, n_leap=args.n_leap,
@@ -228,9 +256,10 @@ def main():
         "mdagm_batch_GBps": roof_b["GBps"],
         "mdagm_from_phi_GBps": roof_phi["GBps"],
         "roofline": {"bound": "hbm", "achieved": roof["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": roof["GBps"] / HBM_PEAK_GBS, "traffic": None,
+                     "frac": roof["GBps"] / HBM_PEAK_GBS, "traffic": traffic_of(L, Lt, roof["nrhs"]),
                      "kernel": roof["kernel"], "nrhs": roof["nrhs"], "bytes_per_launch": roof["bytes_per_launch"],
                      "ms_per_launch": roof["ms_per_launch"]},
+        "tilings": tilings,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -57,6 +57,13 @@ const char* qf_last_error(void);
  * counterpart in the reference, whose kernels take BLOCK_SIZE=(4,8)
  * (hmc_sampler_batch.py:111). */
 int qf_set_tuning(int prec, int by, int t, int tb);
+/* The same for one lattice shape (nrhs = 0: any batch size); looked up
+ * before the per-precision setting.  Written by the autotuner. */
+int qf_set_tuning_shape(int prec, int L, int Ltau, int nrhs, int by, int t, int tb);
+/* Forget every tuning entry (back to the built-in defaults). */
+int qf_clear_tuning(void);
+/* out4 = {patch height, tau slices per operator work-group, tau slices per
+ * update work-group, operator work-groups per right-hand side}. */
 int qf_get_tiling(int prec, int nrhs, int L, int Ltau, int* out4);
 
 /* y = op(phi) x for nrhs right-hand sides.  op = QF_OP_M: get_M_sparse's M

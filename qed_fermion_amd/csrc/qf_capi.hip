@@ -56,7 +56,13 @@ int fail(int code, const char* fmt, ...) {
 struct Tuning {
     int by = 0, t = 0, tb = 0;   // 0 = automatic
 };
-Tuning g_tune[2];                // per precision
+Tuning g_tune[2];                // per precision: applies to every shape
+// Per-shape entries (set by the autotuner); nrhs = 0 matches any batch.
+struct ShapeTuning {
+    int prec, L, Lt, nrhs;
+    Tuning tu;
+};
+std::vector<ShapeTuning> g_shape_tune;
 std::mutex g_tune_mu;
 
 // Work-groups of up to 512 threads get 256 VGPRs per lane; only the 2x2
@@ -70,33 +76,54 @@ struct Tiling {
     int TB, nsx, nB;   // update kernel
 };
 
-int pick_by(int L, int prec) {
-    int want;
-    {
-        std::lock_guard<std::mutex> lk(g_tune_mu);
-        want = g_tune[prec].by;
+// Tuning that applies to (prec, L, Lt, nrhs): exact shape, then any batch
+// size, then the per-precision setting.
+Tuning find_tuning(int prec, int L, int Lt, int nrhs) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const ShapeTuning* any = nullptr;
+    for (const auto& e : g_shape_tune) {
+        if (e.prec != prec || e.L != L || e.Lt != Lt) continue;
+        if (e.nrhs == nrhs) return e.tu;
+        if (e.nrhs == 0) any = &e;
     }
-    if (want && by_ok(L, want)) return want;
-    for (int by : {2, 4, 6})
-        if (by_ok(L, by)) return by;
+    return any ? any->tu : g_tune[prec];
+}
+
+// Default patch height: complex64 kernels fit the 2x4 patch in 256 VGPRs
+// without spilling (more independent work per exchange barrier); complex128
+// kernels only fit the 2x2 patch.  Patches that need the 1024-thread
+// work-group come last.
+int pick_by_default(int L, int prec) {
+    const int order32[3] = {4, 2, 6}, order64[3] = {2, 4, 6};
+    const int* order = prec == QF_PREC_F64 ? order64 : order32;
+    for (int k = 0; k < 3; ++k)
+        if (by_ok(L, order[k]) && nthreads(L, order[k]) <= 512) return order[k];
+    for (int k = 0; k < 3; ++k)
+        if (by_ok(L, order[k])) return order[k];
     return 0;
+}
+
+int pick_by(int L, int prec, int want = 0) {
+    if (want && by_ok(L, want)) return want;
+    return pick_by_default(L, prec);
 }
 
 Tiling tiling(int prec, int nrhs, int L, int Lt) {
     Tiling t;
-    Tuning tu;
-    {
-        std::lock_guard<std::mutex> lk(g_tune_mu);
-        tu = g_tune[prec];
-    }
-    t.by = pick_by(L, prec);
+    const Tuning tu = find_tuning(prec, L, Lt, nrhs);
+    t.by = pick_by(L, prec, tu.by);
     const long long units = (long long)nrhs * Lt;
-    t.T = tu.t ? tu.t : (int)std::max(1LL, std::min(32LL, units / 1024));
+    // tau slices per operator work-group: about 2048 work-groups in flight
+    // over the batch, at least 4 slices so the per-run prologue (one extra B
+    // application) stays small, at most 16
+    t.T = tu.t ? tu.t : (int)std::max(4LL, std::min(16LL, units / 2048));
     t.T = std::min(t.T, Lt);
     t.nA = (Lt + t.T - 1) / t.T;
     const int Vs = L * L;
     t.nsx = (Vs + 255) / 256;
-    t.TB = tu.tb ? tu.tb : (int)std::max(2LL, std::min(64LL, units * t.nsx / 1024));
+    // tau slices per update work-group: the 13-slice preconditioner window
+    // re-reads 12 halo slices per work-group, so prefer long runs (>= 16)
+    t.TB = tu.tb ? tu.tb : (int)std::max(16LL, std::min(64LL, units * t.nsx / 1024));
     t.TB = std::min(t.TB, Lt);
     t.nB = t.nsx * ((Lt + t.TB - 1) / t.TB);
     return t;
@@ -128,7 +155,7 @@ int check_lattice(int L, int Lt, int nrhs) {
     if (L < 2 || (L & 1)) return fail(QF_ERR_ODD_L, "L=%d: the checkerboard bond groups need an even L >= 2", L);
     if (Lt < 1) return fail(QF_ERR_INVALID, "Ltau=%d must be >= 1", Lt);
     if (nrhs < 1 || nrhs > 65535) return fail(QF_ERR_INVALID, "nrhs=%d out of range [1, 65535]", nrhs);
-    if (pick_by(L, 0) == 0) return fail(QF_ERR_UNSUPPORTED, "L=%d: no patch shape fits a 512/1024-thread work-group", L);
+    if (pick_by_default(L, 0) == 0) return fail(QF_ERR_UNSUPPORTED, "L=%d: no patch shape fits a 512/1024-thread work-group", L);
     return QF_OK;
 }
 
@@ -152,23 +179,37 @@ int set_lds(KernelT kern, size_t bytes) {
 inline dim3 blk_of(int L, int by) { return dim3((unsigned)(((nthreads(L, by) + 63) / 64) * 64)); }
 
 // ---------------------------------------------------------------- launchers
-template <typename R, int BY, int MODE>
-int launch_mdm_by(const MdmArgs<R>& a, int nrhs, hipStream_t s) {
+template <typename R, int BY, int MODE, bool HOPT>
+int launch_mdm_k(const MdmArgs<R>& a, int nrhs, hipStream_t s) {
     const size_t lds = lds_bytes<R>(a.L, 2);   // two vectors per exchange (tau pipeline)
     const dim3 grid((a.Lt + a.T - 1) / a.T, nrhs), blk = blk_of(a.L, BY);
     if (blk.x <= 512) {
-        auto k = mdagm_kernel<R, BY, MODE, 512>;
+        auto k = mdagm_kernel<R, BY, MODE, 512, HOPT>;
         int rc = set_lds(k, lds);
         if (rc) return rc;
         hipLaunchKernelGGL(k, grid, blk, lds, s, a);
     } else if constexpr (BY == 2) {
-        auto k = mdagm_kernel<R, BY, MODE, 1024>;
+        auto k = mdagm_kernel<R, BY, MODE, 1024, HOPT>;
         int rc = set_lds(k, lds);
         if (rc) return rc;
         hipLaunchKernelGGL(k, grid, blk, lds, s, a);
     }
     QF_LAUNCHED();
     return QF_OK;
+}
+
+// CG step A always reads the solve's hop table; the FP64 residual of the
+// reliable updates reads phi; plain applications take either.
+template <typename R, int BY, int MODE>
+int launch_mdm_by(const MdmArgs<R>& a, int nrhs, hipStream_t s) {
+    if constexpr (MODE == MDM_CG) {
+        if (!a.hop) return fail(QF_ERR_INVALID, "CG step without a hop table");
+        return launch_mdm_k<R, BY, MODE, true>(a, nrhs, s);
+    } else if constexpr (MODE == MDM_RESID) {
+        return launch_mdm_k<R, BY, MODE, false>(a, nrhs, s);
+    } else {
+        return a.hop ? launch_mdm_k<R, BY, MODE, true>(a, nrhs, s) : launch_mdm_k<R, BY, MODE, false>(a, nrhs, s);
+    }
 }
 
 template <typename R, int MODE>
@@ -452,6 +493,31 @@ int qf_set_tuning(int prec, int by, int t, int tb) {
     g_tune[prec].by = by;
     g_tune[prec].t = t;
     g_tune[prec].tb = tb;
+    return QF_OK;
+}
+
+int qf_set_tuning_shape(int prec, int L, int Ltau, int nrhs, int by, int t, int tb) {
+    if (prec != QF_PREC_F32 && prec != QF_PREC_F64) return fail(QF_ERR_INVALID, "prec %d", prec);
+    if (by != 0 && !by_ok(L, by)) return fail(QF_ERR_INVALID, "patch height %d does not fit L=%d", by, L);
+    if (t < 0 || tb < 0 || nrhs < 0 || L < 2 || Ltau < 1) return fail(QF_ERR_INVALID, "bad shape or tile size");
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (auto& e : g_shape_tune)
+        if (e.prec == prec && e.L == L && e.Lt == Ltau && e.nrhs == nrhs) {
+            e.tu.by = by; e.tu.t = t; e.tu.tb = tb;
+            return QF_OK;
+        }
+    ShapeTuning e;
+    e.prec = prec; e.L = L; e.Lt = Ltau; e.nrhs = nrhs;
+    e.tu.by = by; e.tu.t = t; e.tu.tb = tb;
+    g_shape_tune.push_back(e);
+    return QF_OK;
+}
+
+int qf_clear_tuning(void) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_shape_tune.clear();
+    g_tune[0] = Tuning();
+    g_tune[1] = Tuning();
     return QF_OK;
 }
 

@@ -60,6 +60,44 @@ template <typename R> __device__ __forceinline__ cplx<R> pair_addc(cplx<R> v, cp
     return mk<R>(fma(h.y, w.y, fma(h.x, w.x, v.x)), fma(-h.y, w.x, fma(h.x, w.y, v.y)));
 }
 
+// Raw buffer loads (buffer_load with a per-lane VGPR byte offset and a
+// uniform SGPR byte offset): a whole patch row or slice is addressed by adding
+// uniform offsets on the scalar unit, so the VALU does no address arithmetic.
+// The resource covers [p, p + bytes); every access here is in range.
+// Stores are NOT done this way: a buffer store of more than 64 bits whose
+// soffset is an SGPR is treated by the compiler as free of the store-data
+// hazard, and on gfx950 the next VALU write of its data VGPRs corrupted the
+// stored value (lanes 12-15 of each 16-lane group).  Stores use global
+// instructions (uniform 64-bit base + per-lane 32-bit offset), which the
+// compiler's hazard model covers.
+typedef unsigned int qf_u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int qf_u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+// one complex value
+template <typename R> __device__ __forceinline__ cplx<R> buf_ld1(__amdgpu_buffer_rsrc_t rs, unsigned vo, unsigned so);
+template <> __device__ __forceinline__ float2 buf_ld1<float>(__amdgpu_buffer_rsrc_t rs, unsigned vo, unsigned so) {
+    const qf_u32x2 q = __builtin_amdgcn_raw_buffer_load_b64(rs, vo, so, 0);
+    return make_float2(__uint_as_float(q.x), __uint_as_float(q.y));
+}
+template <> __device__ __forceinline__ double2 buf_ld1<double>(__amdgpu_buffer_rsrc_t rs, unsigned vo, unsigned so) {
+    const qf_u32x4 q = __builtin_amdgcn_raw_buffer_load_b128(rs, vo, so, 0);
+    return make_double2(__hiloint2double(q.y, q.x), __hiloint2double(q.w, q.z));
+}
+// two adjacent complex values
+template <typename R> __device__ __forceinline__ void buf_ld2(__amdgpu_buffer_rsrc_t rs, unsigned vo, unsigned so, cplx<R>& a, cplx<R>& b);
+template <> __device__ __forceinline__ void buf_ld2<float>(__amdgpu_buffer_rsrc_t rs, unsigned vo, unsigned so, float2& a, float2& b) {
+    const qf_u32x4 q = __builtin_amdgcn_raw_buffer_load_b128(rs, vo, so, 0);
+    a = make_float2(__uint_as_float(q.x), __uint_as_float(q.y));
+    b = make_float2(__uint_as_float(q.z), __uint_as_float(q.w));
+}
+template <> __device__ __forceinline__ void buf_ld2<double>(__amdgpu_buffer_rsrc_t rs, unsigned vo, unsigned so, double2& a, double2& b) {
+    a = buf_ld1<double>(rs, vo, so);
+    b = buf_ld1<double>(rs, vo + 16, so);
+}
 // Trotter factor constants: a_1 = t*dtau (group 1), a_{2,3,4} = t*dtau/2
 // (hmc_sampler_batch.py:1508-1550).  ch/sh: cosh/sinh of a_1 (suffix 1) and of
 // a_{2,3,4} (suffix 2).  Normalised form: D_g = cosh(a_g) (I + tanh(a_g) K_g),

@@ -70,22 +70,91 @@ struct MdmArgs {
 
 enum { MDM_PLAIN = 0, MDM_CG = 1, MDM_RESID = 2 };
 
-template <typename R, int BY, int MODE, int NT>
+// tau pipeline of one work-group over slices [t0, t1) of one RHS:
+//   prologue: U = B_{t0-1} x_{t0-1}, W = B_{t0} x_{t0} (one shared exchange
+//             per factor); y_{t0} = x_{t0} + sM U, y_{t0+1} = x_{t0+1} + sM W
+//   step tau: W = B_tau y_{tau+1} and U = B_{tau+1} x_{tau+1} together;
+//             out_tau = y_tau + sD W;  y_{tau+2} = x_{tau+2} + sM U
+// Loads run one step ahead: step tau issues the coefficients of slice tau+2
+// and x_{tau+2} before its factors, so their latency overlaps the 7 exchange
+// barriers.  Three coefficient sets rotate (C_tau, C_{tau+1} in use, C_{tau+2}
+// landing); the loop is unrolled by three so the rotation is free.
+// HOPT: coefficients from the hop table (buffer loads) instead of sincos(phi).
+// Where three coefficient sets do not fit the register budget (mdm_regpf
+// false), step tau loads C_{tau+1} itself and instead touches the cache lines
+// of slice tau+2's coefficients one step ahead, so that load hits L2.
+template <typename R, int BY, int MODE>
+__host__ __device__ constexpr bool mdm_regpf() { return sizeof(R) == 4 && (BY == 2 || (BY == 4 && MODE == MDM_PLAIN)); }
+
+// Touch the 128-byte lines of one slice region [so, so + bytes) of a buffer:
+// lane i loads one dword of lines i, i + nlanes, ...  The dwords are returned
+// so the caller can fold them in later (an unused load would be dropped and a
+// consumed one must not be waited for at once).
+template <int NTOUCH>
+__device__ __forceinline__ void touch_lines(unsigned (&d)[NTOUCH], __amdgpu_buffer_rsrc_t rs, unsigned so, unsigned bytes, unsigned nlanes) {
+#pragma unroll
+    for (int k = 0; k < NTOUCH; ++k) {
+        const unsigned off = min((threadIdx.x + k * nlanes) * 128u, bytes - 4u);
+        d[k] = __builtin_amdgcn_raw_buffer_load_b32(rs, off, so, 0);
+    }
+}
+
+template <typename R, int BY, int MODE, int NT, bool HOPT>
 __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
     extern __shared__ __align__(16) unsigned char qf_smem[];
     cplx<R>* lds = reinterpret_cast<cplx<R>*>(qf_smem);
     __shared__ double red[64];
+    constexpr unsigned S = sizeof(cplx<R>);
     const int rho = blockIdx.y;
     const int L = A.L, Lt = A.Lt, Vs = L * L;
     const int t0 = blockIdx.x * A.T;
     const int t1 = min(t0 + A.T, Lt);
     const Geo g = make_geo<BY>(L, threadIdx.x);
-    const R* phi = A.phi + rho * A.phi_cs;
-    const long long vo = rho * A.vs;
-    R beta = R(0);
+    const PatchOff po = patch_off<R>(g);
+    const unsigned vbytes = (unsigned)Vs * Lt * S;   // one vector
+    const unsigned sstride = (unsigned)Vs * S;       // one slice of a vector
     if constexpr (MODE == MDM_RESID) {
         if (A.ru_flag != nullptr && A.ru_flag[rho] == 0) return;   // uniform over the block
     }
+    const long long vo = rho * A.vs;
+    const R* phi = A.phi + rho * A.phi_cs;
+    const __amdgpu_buffer_rsrc_t rs_h = HOPT ? buf_rsrc(A.hop + rho * A.hop_cs, 2u * vbytes) : buf_rsrc(A.x, 0u);
+    // x-like inputs: MODE_CG reads z and p_old (x = z + beta p_old), else x
+    const bool cg = MODE == MDM_CG;
+    const __amdgpu_buffer_rsrc_t rs_a = buf_rsrc(cg ? (const void*)(A.z + vo) : (const void*)(A.x + vo), vbytes);
+    const __amdgpu_buffer_rsrc_t rs_p = buf_rsrc(cg ? (const void*)(A.pold + vo) : (const void*)(A.x + vo), vbytes);
+    const bool use_p = cg && A.iter > 0;
+
+    auto coef = [&](CoefC<R, BY>& cf, int tau) {
+        if constexpr (HOPT)
+            load_coefc_buf<R, BY>(cf, rs_h, po, (unsigned)tau * 2u * sstride, L);
+        else
+            load_coefc<R, BY>(cf, phi + (long long)tau * 2 * Vs, g, A.hp);
+    };
+    struct Raw { cplx<R> a[2 * BY], p[2 * BY]; };
+    auto ldraw = [&](Raw& w, int tau) {
+        load_patch_buf<R, BY>(w.a, rs_a, po, (unsigned)tau * sstride, L);
+        if (use_p) load_patch_buf<R, BY>(w.p, rs_p, po, (unsigned)tau * sstride, L);
+    };
+    R beta = R(0);
+    auto make = [&](cplx<R> (&v)[2 * BY], const Raw& w) {
+        if (use_p) {
+            axpy_patch<R, BY>(v, w.a, beta, w.p);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 2 * BY; ++k) v[k] = w.a[k];
+        }
+    };
+
+    // prologue loads first: they do not depend on the CG scalars
+    const int tm = (t0 + Lt - 1) % Lt;
+    const int t0n = (t0 + 1 == Lt) ? 0 : t0 + 1;
+    CoefC<R, BY> c0, c1, c2;
+    Raw rw, rx;
+    coef(c2, tm);
+    coef(c0, t0);
+    ldraw(rw, tm);
+    ldraw(rx, t0);
 
     if constexpr (MODE == MDM_CG) {
         // Scalars of the previous iteration, reduced in a fixed order by
@@ -94,18 +163,18 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
         double sums[2];
         const double* src[2] = {A.rz_part + (long long)rho * A.nB, A.rr_part + (long long)rho * A.nB};
         sum_partials<2>(src, A.nB, sums, red);
-        const CgState S = A.st_prev[rho];
+        const CgState St = A.st_prev[rho];
         const double nb = (A.iter == 0) ? sqrt(sums[1]) : A.normb[rho];
-        double act, err, rzv, it = S.iters, bt = 0.0;
-        if (S.act == 0.0) {
-            act = 0.0; err = S.err; rzv = S.rz;
+        double act, err, rzv, it = St.iters, bt = 0.0;
+        if (St.act == 0.0) {
+            act = 0.0; err = St.err; rzv = St.rz;
         } else {
             err = nb > 0.0 ? sqrt(sums[1]) / nb : 0.0;
             const int may = A.stop_flag ? A.stop_flag[rho] : A.may_stop;
             act = (err > A.rtol || !may) ? 1.0 : 0.0;
             rzv = sums[0];
             if (A.iter > 0) {
-                bt = sums[0] / S.rz;
+                bt = sums[0] / St.rz;
                 if (act != 0.0) it += 1.0;
             }
         }
@@ -118,112 +187,110 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
         beta = (R)bt;
     }
 
-    const cplx<R>* x = A.x + vo;
-    const cplx<R>* hop = A.hop ? A.hop + rho * A.hop_cs : nullptr;
-    auto coef = [&](Coef<R, BY>& cf, int tau) {
-        if (hop)
-            load_coef_tab<R, BY>(cf, hop + (long long)tau * 2 * Vs, g);
-        else
-            load_coef<R, BY>(cf, phi + (long long)tau * 2 * Vs, g, A.hp);
-    };
-    auto ldx = [&](int tau, cplx<R> (&v)[2 * BY]) {
-        if constexpr (MODE == MDM_CG) {
-            const long long so = vo + (long long)tau * Vs;
-            cplx<R> zz[2 * BY];
-            load_patch<R, BY>(zz, A.z + so, g);
-            if (A.iter == 0) {
-#pragma unroll
-                for (int k = 0; k < 2 * BY; ++k) v[k] = zz[k];
-            } else {
-                cplx<R> pp[2 * BY];
-                load_patch<R, BY>(pp, A.pold + so, g);
-                axpy_patch<R, BY>(v, zz, beta, pp);
-            }
-        } else {
-            load_patch<R, BY>(v, x + (long long)tau * Vs, g);
-        }
-    };
+    const __amdgpu_buffer_rsrc_t rs_b = buf_rsrc(MODE == MDM_RESID ? (const void*)(A.b + vo) : (const void*)A.x, MODE == MDM_RESID ? vbytes : 0u);
+    cplx<R>* const yout = MODE == MDM_RESID ? nullptr : A.y + vo;
+    cplx<R>* const pnew = MODE == MDM_CG ? A.pnew + vo : nullptr;
+    float2* r_lo = MODE == MDM_RESID ? A.r_lo + vo : nullptr;
 
-    // tau pipeline: the second B of slice tau (W = B_tau y_{tau+1}) and the
-    // first B of slice tau+1 (U = B_{tau+1} x_{tau+1}) are independent, so they
-    // run together and share every exchange barrier (7 per slice, not 14).
-    // The dot product accumulated is ||y||^2 = Re(x^dag M^dag M x) summed over
-    // the run's slices (y = Mx is in registers; no x copy needed).
     int phase = 0;
-    Coef<R, BY> cfc, cfn;
+    const R cn = A.hp.cn;   // apply_B returns B v / cn
     cplx<R> X[2 * BY], Y[2 * BY], Yn[2 * BY], U[2 * BY], W[2 * BY];
-#pragma unroll
-    for (int k = 0; k < 2 * BY; ++k) X[k] = Y[k] = Yn[k] = U[k] = W[k] = mk<R>(0, 0);
     double dsum = 0.0;
 
-    // prologue: U = B_{t0-1} x_{t0-1}, W = B_{t0} x_{t0};  y_{t0}, y_{t0+1}
-    const int tm = (t0 + Lt - 1) % Lt;
-    const int t0n = (t0 + 1 == Lt) ? 0 : t0 + 1;
-    coef(cfn, tm);
-    coef(cfc, t0);
-    ldx(tm, U);
-    ldx(t0, X);
+    make(U, rw);   // x_{t0-1}
+    make(X, rx);   // x_{t0}
     if constexpr (MODE == MDM_CG) {
-        if (g.valid) store_patch<R, BY>(A.pnew + vo + (long long)t0 * Vs, X, g);
+        if (g.valid) store_patch_g<R, BY>(X, pnew + (long long)t0 * Vs, po, L);
     }
 #pragma unroll
     for (int k = 0; k < 2 * BY; ++k) W[k] = X[k];
-    apply_B2<R, BY>(U, cfn, W, cfc, A.hp, lds, g, phase);
-    const R cn = A.hp.cn;   // apply_B returns B v / cn
+    constexpr bool REGPF = mdm_regpf<R, BY, MODE>();
+    constexpr int NTOUCH = (BY * (int)S + 31) / 32;   // 128-byte lines of a hop slice per lane
+    unsigned tacc = 0, td[NTOUCH];
+    // coefficient source of one slice: hop entries or phi
+    const unsigned cslice = HOPT ? 2u * sstride : 2u * (unsigned)Vs * (unsigned)sizeof(R), nlanes = blockDim.x;
+    const __amdgpu_buffer_rsrc_t rs_c = HOPT ? rs_h : buf_rsrc(phi, cslice * (unsigned)Lt);
+    if constexpr (REGPF) {
+        if (t0 + 1 < t1) coef(c1, t0 + 1);
+    } else {
+        touch_lines<NTOUCH>(td, rs_c, (unsigned)t0n * cslice, cslice, nlanes);
+    }
+    ldraw(rx, t0n);   // x_{t0+1}
+    apply_B2c<R, BY>(U, c2, W, c0, lds, g, phase);
     axpy_patch<R, BY>(Y, X, sgnM(t0) * cn, U);
-    ldx(t0n, X);
+    make(X, rx);
     axpy_patch<R, BY>(Yn, X, sgnM(t0n) * cn, W);
 
-    for (int tau = t0; tau < t1; ++tau) {
-        const int tn = (tau + 1 == Lt) ? 0 : tau + 1;
+    // one step; cA = C_tau, cB = C_{tau+1}, cC receives C_{tau+2}
+    auto step = [&](int tau, CoefC<R, BY>& cA, CoefC<R, BY>& cB, CoefC<R, BY>& cC) {
         const bool more = tau + 1 < t1;
+        const int tn = (tau + 1 == Lt) ? 0 : tau + 1;
+        const int tnn = (tn + 1 == Lt) ? 0 : tn + 1;
+        cplx<R> bb[2 * BY];
+        if constexpr (MODE == MDM_RESID) load_patch_buf<R, BY>(bb, rs_b, po, (unsigned)tau * sstride, L);
 #pragma unroll
         for (int k = 0; k < 2 * BY; ++k) W[k] = Yn[k];
         if (more) {
-            coef(cfn, tn);
 #pragma unroll
             for (int k = 0; k < 2 * BY; ++k) U[k] = X[k];
-            apply_B2<R, BY>(W, cfc, U, cfn, A.hp, lds, g, phase);   // W = B_tau y_{tau+1}, U = B_{tau+1} x_{tau+1}
+            if constexpr (MODE == MDM_CG) {
+                if (g.valid) store_patch_g<R, BY>(X, pnew + (long long)tn * Vs, po, L);
+            }
+            if constexpr (REGPF) {
+                if (tau + 2 < t1) coef(cC, tnn);
+            } else {
+                coef(cB, tn);
+#pragma unroll
+                for (int k = 0; k < NTOUCH; ++k) tacc ^= td[k];
+                if (tau + 2 < t1) touch_lines<NTOUCH>(td, rs_c, (unsigned)tnn * cslice, cslice, nlanes);
+            }
+            ldraw(rx, tnn);
+            apply_B2c<R, BY>(W, cA, U, cB, lds, g, phase);   // W = B_tau y_{tau+1}, U = B_{tau+1} x_{tau+1}
         } else {
-            apply_B<R, BY>(W, cfc, A.hp, lds, g, phase);
+            apply_Bc<R, BY>(W, cA, lds, g, phase);
         }
         cplx<R> Z[2 * BY];
         axpy_patch<R, BY>(Z, Y, sgnD(tau, Lt) * cn, W);
         if (g.valid) {
-            const long long so = vo + (long long)tau * Vs;
             if constexpr (MODE == MDM_RESID) {
-                cplx<R> bb[2 * BY];
-                load_patch<R, BY>(bb, A.b + so, g);
 #pragma unroll
                 for (int k = 0; k < 2 * BY; ++k) {
-                    const R rx = bb[k].x - Z[k].x, ry = bb[k].y - Z[k].y;
-                    dsum += (double)rx * rx + (double)ry * ry;
-                    Z[k] = mk<R>(rx, ry);
+                    const R rx_ = bb[k].x - Z[k].x, ry_ = bb[k].y - Z[k].y;
+                    dsum += (double)rx_ * rx_ + (double)ry_ * ry_;
+                    Z[k] = mk<R>(rx_, ry_);
                 }
-                float2* rp = A.r_lo + so + (g.y0) * L + 2 * g.a;
+                float2* rp = r_lo + (long long)tau * Vs + g.y0 * L + 2 * g.a;
 #pragma unroll
-                for (int r = 0; r < BY; ++r) {
-                    rp[r * L] = make_float2((float)Z[2 * r].x, (float)Z[2 * r].y);
-                    rp[r * L + 1] = make_float2((float)Z[2 * r + 1].x, (float)Z[2 * r + 1].y);
-                }
+                for (int r = 0; r < BY; ++r)
+                    *reinterpret_cast<float4*>(rp + r * L) =
+                        make_float4((float)Z[2 * r].x, (float)Z[2 * r].y, (float)Z[2 * r + 1].x, (float)Z[2 * r + 1].y);
             } else {
-                store_patch<R, BY>(A.y + so, Z, g);
+                store_patch_g<R, BY>(Z, yout + (long long)tau * Vs, po, L);
                 dsum += redot_patch<R, BY>(Y, Y);
             }
         }
         if (more) {
-            const int tnn = (tn + 1 == Lt) ? 0 : tn + 1;
 #pragma unroll
             for (int k = 0; k < 2 * BY; ++k) Y[k] = Yn[k];
-            if constexpr (MODE == MDM_CG) {
-                if (g.valid) store_patch<R, BY>(A.pnew + vo + (long long)tn * Vs, X, g);
-            }
-            ldx(tnn, X);
-            axpy_patch<R, BY>(Yn, X, sgnM(tnn) * cn, U);        // y_{tau+2}
-            cfc = cfn;
+            make(X, rx);                                          // x_{tau+2}
+            axpy_patch<R, BY>(Yn, X, sgnM(tnn) * cn, U);          // y_{tau+2}
         }
+    };
+    for (int tau = t0;;) {
+        step(tau, c0, c1, c2);
+        if (++tau >= t1) break;
+        step(tau, c1, c2, c0);
+        if (++tau >= t1) break;
+        step(tau, c2, c0, c1);
+        if (++tau >= t1) break;
     }
 
+    if constexpr (!REGPF) {
+        // fold the touched dwords in; T < 0 never holds, so this only keeps the loads alive
+#pragma unroll
+        for (int k = 0; k < NTOUCH; ++k) tacc ^= td[k];
+        if (A.T < 0 && tacc == 0x7fc00001u) dsum = 0.0;
+    }
     if (A.dot_part != nullptr) {
         double v[1] = {dsum};
         block_sum<1>(v, red);

@@ -142,16 +142,107 @@ __device__ __forceinline__ void load_coef_tab(Coef<R, BY>& cf, const cplx<R>* __
     cf.ey[1][0] = hop_s[2 * (ym * L + x0 + 1) + 1];
 }
 
+// Per-lane byte offsets inside one slice for buffer addressing: the patch
+// origin in a vector slice, and the hop-table entries a patch needs.  Row and
+// slice offsets are uniform and go in the scalar offset.
+struct PatchOff {
+    unsigned v;    // vector: site (2a, y0)
+    unsigned h;    // hop: sites (2a, y0) and (2a+1, y0), both mu
+    unsigned hm;   // hop: x-entry of (xm, y0), xm = 2a-1 mod L
+    unsigned hy;   // hop: y-entry of (2a, ym), ym = y0-1 mod L
+};
+
+template <typename R>
+__device__ __forceinline__ PatchOff patch_off(const Geo& g) {
+    constexpr unsigned S = sizeof(cplx<R>);
+    const int L = g.L, x0 = 2 * g.a;
+    const int xm = x0 == 0 ? L - 1 : x0 - 1;
+    const int ym = g.y0 == 0 ? L - 1 : g.y0 - 1;
+    PatchOff o;
+    o.v = (unsigned)(g.y0 * L + x0) * S;
+    o.h = (unsigned)(g.y0 * L + x0) * 2 * S;
+    o.hm = (unsigned)(g.y0 * L + xm) * 2 * S;
+    o.hy = (unsigned)(ym * L + x0) * 2 * S + S;
+    return o;
+}
+
+// Patch of one vector slice; so = byte offset of the slice.
+template <typename R, int BY>
+__device__ __forceinline__ void load_patch_buf(cplx<R> (&v)[2 * BY], __amdgpu_buffer_rsrc_t rs, const PatchOff& o, unsigned so, int L) {
+    constexpr unsigned S = sizeof(cplx<R>);
+#pragma unroll
+    for (int r = 0; r < BY; ++r) buf_ld2<R>(rs, o.v, so + (unsigned)(r * L) * S, v[2 * r], v[2 * r + 1]);
+}
+
+// Store of the patch into one vector slice (slice: uniform base pointer).
+template <typename R, int BY>
+__device__ __forceinline__ void store_patch_g(const cplx<R> (&v)[2 * BY], cplx<R>* slice, const PatchOff& o, int L) {
+#pragma unroll
+    for (int r = 0; r < BY; ++r) {
+        cplx<R>* row = slice + r * L;   // uniform
+        st_pair<R>(reinterpret_cast<cplx<R>*>(reinterpret_cast<char*>(row) + o.v), v[2 * r], v[2 * r + 1]);
+    }
+}
+
+// Coefficients of one slice from the hop table; so = byte offset of the slice
+// (tau * 2 * Vs * sizeof(cplx)).
+template <typename R, int BY>
+__device__ __forceinline__ void load_coef_buf(Coef<R, BY>& cf, __amdgpu_buffer_rsrc_t rs, const PatchOff& o, unsigned so, int L) {
+    constexpr unsigned S = sizeof(cplx<R>);
+#pragma unroll
+    for (int r = 0; r < BY; ++r) {
+        const unsigned row = so + (unsigned)(r * L) * 2 * S;
+        cplx<R> h0y, h1y;
+        buf_ld2<R>(rs, o.h, row, cf.ex[r][1], h0y);
+        buf_ld2<R>(rs, o.h + 2 * S, row, cf.ex[r][2], h1y);
+        cf.ex[r][0] = buf_ld1<R>(rs, o.hm, row);
+        cf.ey[0][r + 1] = h0y;
+        cf.ey[1][r + 1] = h1y;
+    }
+    cf.ey[0][0] = buf_ld1<R>(rs, o.hy, so);
+    cf.ey[1][0] = buf_ld1<R>(rs, o.hy + 2 * S, so);
+}
+
 // LDS exchange buffer: two alternating halves, each NV * (L/2)*L complex
 // (NV vectors exchanged together).  Callers size the dynamic LDS with
-// lds_bytes<R>(L, NV).  Within a vector's sub-buffer the edge values are kept
-// in four planes indexed by the patch column a, so a wave's writes and reads
-// are contiguous (bank-conflict free):
+// lds_bytes<R>(L, NV).  Edge values are kept in four planes indexed by the
+// patch column a, so a wave's writes and reads are contiguous (bank-conflict
+// free); the NV vectors' values of one site share one slot of NV complex, so
+// one 16-byte (NV = 2, complex64) LDS access moves both:
 //   x-groups: EL[row][a] (cx=0 site), ER[row][a] (cx=1 site), row = y >> 1
 //   y-groups: BOT[b][a] (r=0 site), TOP[b][a] (r=BY-1 site)
 __host__ __device__ constexpr int xstride(int L) { return (L >> 1) * L; }
 template <typename R>
 inline size_t lds_bytes(int L, int NV) { return (size_t)2 * NV * xstride(L) * sizeof(cplx<R>); }
+
+// Write / read one slot of NV complex values (vector n's value at slot[n]).
+template <typename R, int NV>
+struct SlotIO {
+    template <typename F>
+    __device__ __forceinline__ static void put(cplx<R>* slot, F&& val) {
+#pragma unroll
+        for (int n = 0; n < NV; ++n) slot[n] = val(n);
+    }
+    template <typename F>
+    __device__ __forceinline__ static void get(const cplx<R>* slot, F&& sink) {
+#pragma unroll
+        for (int n = 0; n < NV; ++n) sink(n, slot[n]);
+    }
+};
+template <>
+struct SlotIO<float, 2> {
+    template <typename F>
+    __device__ __forceinline__ static void put(float2* slot, F&& val) {
+        const float2 a = val(0), b = val(1);
+        *reinterpret_cast<float4*>(slot) = make_float4(a.x, a.y, b.x, b.y);
+    }
+    template <typename F>
+    __device__ __forceinline__ static void get(const float2* slot, F&& sink) {
+        const float4 q = *reinterpret_cast<const float4*>(slot);
+        sink(0, make_float2(q.x, q.y));
+        sink(1, make_float2(q.z, q.w));
+    }
+};
 
 // Partner values of the sites that cross a patch edge for group G.  Writes the
 // patch's edge values, barriers, reads the neighbours' edge values.  pr[] gets
@@ -162,45 +253,35 @@ template <typename R, int BY, int G, int NV>
 __device__ __forceinline__ void exchange(cplx<R> (*v)[2 * BY], cplx<R> (*pr)[BY], cplx<R>* lds, const Geo& g, int& phase) {
     const int L = g.L;
     const int na = L >> 1;
-    const int half = xstride(L);                 // per-vector sub-buffer
-    const int plane = half >> 1;                 // na * L/2
-    cplx<R>* buf = lds + (phase & 1) * NV * half;
+    const int plane = xstride(L) >> 1;           // na * L/2 slots
+    cplx<R>* buf = lds + (phase & 1) * NV * xstride(L);
+    using IO = SlotIO<R, NV>;
     if constexpr (G == 1 || G == 3) {
         constexpr int rc = (G == 1) ? 1 : 0;   // edge rows of group G have r % 2 == rc
 #pragma unroll
-        for (int n = 0; n < NV; ++n)
-#pragma unroll
-            for (int r = rc; r < BY; r += 2) {
-                const int o = ((g.y0 + r) >> 1) * na + g.a;
-                buf[n * half + o] = v[n][2 * r];
-                buf[n * half + plane + o] = v[n][2 * r + 1];
-            }
+        for (int r = rc; r < BY; r += 2) {
+            const int o = ((g.y0 + r) >> 1) * na + g.a;
+            IO::put(buf + o * NV, [&](int n) { return v[n][2 * r]; });
+            IO::put(buf + (plane + o) * NV, [&](int n) { return v[n][2 * r + 1]; });
+        }
         lds_barrier();
         const int am = g.a == 0 ? na - 1 : g.a - 1, ap = g.a + 1 == na ? 0 : g.a + 1;
 #pragma unroll
-        for (int n = 0; n < NV; ++n)
-#pragma unroll
-            for (int r = rc, j = 0; r < BY; r += 2, ++j) {
-                const int rowo = ((g.y0 + r) >> 1) * na;
-                pr[n][2 * j] = buf[n * half + plane + rowo + am];
-                pr[n][2 * j + 1] = buf[n * half + rowo + ap];
-            }
+        for (int r = rc, j = 0; r < BY; r += 2, ++j) {
+            const int rowo = ((g.y0 + r) >> 1) * na;
+            IO::get(buf + (plane + rowo + am) * NV, [&](int n, cplx<R> c) { pr[n][2 * j] = c; });
+            IO::get(buf + (rowo + ap) * NV, [&](int n, cplx<R> c) { pr[n][2 * j + 1] = c; });
+        }
     } else {
         constexpr int cc = (G == 2) ? 1 : 0;   // edge column of group G
         const int yplane = g.nb * na;
-#pragma unroll
-        for (int n = 0; n < NV; ++n) {
-            const int o = g.b * na + g.a;
-            buf[n * half + o] = v[n][cc];
-            buf[n * half + yplane + o] = v[n][2 * (BY - 1) + cc];
-        }
+        const int o = g.b * na + g.a;
+        IO::put(buf + o * NV, [&](int n) { return v[n][cc]; });
+        IO::put(buf + (yplane + o) * NV, [&](int n) { return v[n][2 * (BY - 1) + cc]; });
         lds_barrier();
         const int bm = g.b == 0 ? g.nb - 1 : g.b - 1, bp = g.b + 1 == g.nb ? 0 : g.b + 1;
-#pragma unroll
-        for (int n = 0; n < NV; ++n) {
-            pr[n][0] = buf[n * half + yplane + bm * na + g.a];
-            pr[n][1] = buf[n * half + bp * na + g.a];
-        }
+        IO::get(buf + (yplane + bm * na + g.a) * NV, [&](int n, cplx<R> c) { pr[n][0] = c; });
+        IO::get(buf + (bp * na + g.a) * NV, [&](int n, cplx<R> c) { pr[n][1] = c; });
     }
     ++phase;
 }
@@ -320,6 +401,168 @@ __device__ __forceinline__ void apply_B_partial(cplx<R> (&v)[2 * BY], int nfac, 
             default: factor<R, BY, 4>(v, cf, hp, lds, g, phase); break;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Compact form for the operator kernels: every bond coefficient is held once,
+// by the patch that owns the bond's first site.  For an edge bond (i, j) the
+// owner of i sends conj(h) u_i instead of u_i, so the partner only adds what
+// it receives (u_j += conj(h) u_i) and needs no copy of h; the owner reads the
+// raw u_j and applies h itself.  In an edge row of an x-group the cx=0 site is
+// the second site of the bond from the left neighbour and the cx=1 site the
+// first site of the bond to the right; in the edge column of a y-group the
+// r=0 site is the second site of the bond from below and r=BY-1 the first
+// site of the bond upwards.
+//   ex[r][0] = th e^{i phi_x(2a,   y0+r)}   (bond inside the patch)
+//   ex[r][1] = th e^{i phi_x(2a+1, y0+r)}   (bond to the right neighbour)
+//   ey[c][r] = th e^{i phi_y(2a+c, y0+r)}   (bond upwards from row r)
+template <typename R, int BY>
+struct CoefC {
+    cplx<R> ex[BY][2];
+    cplx<R> ey[2][BY];
+};
+
+template <typename R, int BY>
+__device__ __forceinline__ void load_coefc_buf(CoefC<R, BY>& cf, __amdgpu_buffer_rsrc_t rs, const PatchOff& o, unsigned so, int L) {
+    constexpr unsigned S = sizeof(cplx<R>);
+#pragma unroll
+    for (int r = 0; r < BY; ++r) {
+        const unsigned row = so + (unsigned)(r * L) * 2 * S;
+        buf_ld2<R>(rs, o.h, row, cf.ex[r][0], cf.ey[0][r]);
+        buf_ld2<R>(rs, o.h + 2 * S, row, cf.ex[r][1], cf.ey[1][r]);
+    }
+}
+
+// From phi (sincos), scaled by tanh(a_g) for the normalised factors.
+template <typename R, int BY>
+__device__ __forceinline__ void load_coefc(CoefC<R, BY>& cf, const R* __restrict__ phi_s, const Geo& g, const Hop<R>& hp) {
+    const int L = g.L;
+    const cplx<R>* ps = reinterpret_cast<const cplx<R>*>(phi_s);
+    const int x0 = 2 * g.a;
+#pragma unroll
+    for (int r = 0; r < BY; ++r) {
+        const int y = g.y0 + r;
+        const cplx<R> p0 = ps[y * L + x0];
+        const cplx<R> p1 = ps[y * L + x0 + 1];
+        const R t_in = (r & 1) ? hp.th2 : hp.th1;   // g1 (r even) or g3 (r odd)
+        const R t_ed = (r & 1) ? hp.th1 : hp.th2;
+        cf.ex[r][0] = phase<R>(t_in, p0.x);
+        cf.ex[r][1] = phase<R>(t_ed, p1.x);
+        cf.ey[0][r] = phase<R>(hp.th2, p0.y);
+        cf.ey[1][r] = phase<R>(hp.th2, p1.y);
+    }
+}
+
+// conj(h) * u
+template <typename R> __device__ __forceinline__ cplx<R> cjmul_h(cplx<R> h, cplx<R> u) {
+    return mk<R>(fma(h.x, u.x, h.y * u.y), fma(h.x, u.y, -h.y * u.x));
+}
+
+// One normalised factor (I + th K_G) on NV vectors with coefficient sets
+// cf[n], one exchange barrier for all of them.  The two alternating buffers
+// are 2 * xstride(L) slots apart whatever NV is (LDS sized lds_bytes(L, 2)),
+// so factors on one and on two vectors can follow each other without a
+// write-after-read overlap.
+template <typename R, int BY, int G, int NV>
+__device__ __forceinline__ void factor_c(cplx<R> (*v)[2 * BY], const CoefC<R, BY>* const* cf, cplx<R>* lds, const Geo& g, int& phase) {
+    static_assert(NV == 1 || NV == 2, "one or two vectors per exchange");
+    const int L = g.L;
+    const int na = L >> 1;
+    const int plane = xstride(L) >> 1;
+    cplx<R>* buf = lds + (phase & 1) * 2 * xstride(L);
+    using IO = SlotIO<R, NV>;
+    if constexpr (G == 1 || G == 3) {
+        constexpr int rc = (G == 1) ? 1 : 0;   // edge rows have r % 2 == rc
+#pragma unroll
+        for (int r = rc; r < BY; r += 2) {
+            const int o = ((g.y0 + r) >> 1) * na + g.a;
+            IO::put(buf + o * NV, [&](int n) { return v[n][2 * r]; });                                   // EL: raw
+            IO::put(buf + (plane + o) * NV, [&](int n) { return cjmul_h<R>(cf[n]->ex[r][1], v[n][2 * r + 1]); });   // ER: premultiplied
+        }
+        // inside pairs meanwhile
+#pragma unroll
+        for (int n = 0; n < NV; ++n)
+#pragma unroll
+            for (int r = 1 - rc; r < BY; r += 2) {
+                const cplx<R> v0 = v[n][2 * r], v1 = v[n][2 * r + 1], h = cf[n]->ex[r][0];
+                v[n][2 * r] = pair_add<R>(v0, h, v1);
+                v[n][2 * r + 1] = pair_addc<R>(v1, h, v0);
+            }
+        lds_barrier();
+        const int am = g.a == 0 ? na - 1 : g.a - 1, ap = g.a + 1 == na ? 0 : g.a + 1;
+#pragma unroll
+        for (int r = rc; r < BY; r += 2) {
+            const int rowo = ((g.y0 + r) >> 1) * na;
+            IO::get(buf + (plane + rowo + am) * NV, [&](int n, cplx<R> p) { v[n][2 * r] = cadd<R>(v[n][2 * r], p); });
+            IO::get(buf + (rowo + ap) * NV, [&](int n, cplx<R> w) { v[n][2 * r + 1] = pair_add<R>(v[n][2 * r + 1], cf[n]->ex[r][1], w); });
+        }
+    } else {
+        constexpr int cc = (G == 2) ? 1 : 0;   // edge column
+        constexpr int ci = 1 - cc;             // inside column
+        const int yplane = g.nb * na;
+        const int o = g.b * na + g.a;
+        IO::put(buf + o * NV, [&](int n) { return v[n][cc]; });                                                         // BOT: raw
+        IO::put(buf + (yplane + o) * NV, [&](int n) { return cjmul_h<R>(cf[n]->ey[cc][BY - 1], v[n][2 * (BY - 1) + cc]); });   // TOP
+#pragma unroll
+        for (int n = 0; n < NV; ++n) {
+#pragma unroll
+            for (int r = 0; r < BY; r += 2) {   // inside column: pairs (r, r+1), r even
+                const cplx<R> v0 = v[n][2 * r + ci], v1 = v[n][2 * (r + 1) + ci], h = cf[n]->ey[ci][r];
+                v[n][2 * r + ci] = pair_add<R>(v0, h, v1);
+                v[n][2 * (r + 1) + ci] = pair_addc<R>(v1, h, v0);
+            }
+#pragma unroll
+            for (int r = 1; r + 1 < BY; r += 2) {   // edge column: pairs (r, r+1), r odd
+                const cplx<R> v0 = v[n][2 * r + cc], v1 = v[n][2 * (r + 1) + cc], h = cf[n]->ey[cc][r];
+                v[n][2 * r + cc] = pair_add<R>(v0, h, v1);
+                v[n][2 * (r + 1) + cc] = pair_addc<R>(v1, h, v0);
+            }
+        }
+        lds_barrier();
+        const int bm = g.b == 0 ? g.nb - 1 : g.b - 1, bp = g.b + 1 == g.nb ? 0 : g.b + 1;
+        IO::get(buf + (yplane + bm * na + g.a) * NV, [&](int n, cplx<R> p) { v[n][cc] = cadd<R>(v[n][cc], p); });
+        IO::get(buf + (bp * na + g.a) * NV, [&](int n, cplx<R> w) {
+            v[n][2 * (BY - 1) + cc] = pair_add<R>(v[n][2 * (BY - 1) + cc], cf[n]->ey[cc][BY - 1], w);
+        });
+    }
+    ++phase;
+}
+
+// v1 <- B(c1) v1 / cn and v2 <- B(c2) v2 / cn with 7 barriers.
+template <typename R, int BY>
+__device__ __forceinline__ void apply_B2c(cplx<R> (&v1)[2 * BY], const CoefC<R, BY>& c1, cplx<R> (&v2)[2 * BY], const CoefC<R, BY>& c2,
+                                          cplx<R>* lds, const Geo& g, int& phase) {
+    cplx<R> vv[2][2 * BY];
+#pragma unroll
+    for (int k = 0; k < 2 * BY; ++k) { vv[0][k] = v1[k]; vv[1][k] = v2[k]; }
+    const CoefC<R, BY>* cf[2] = {&c1, &c2};
+    factor_c<R, BY, 4, 2>(vv, cf, lds, g, phase);
+    factor_c<R, BY, 3, 2>(vv, cf, lds, g, phase);
+    factor_c<R, BY, 2, 2>(vv, cf, lds, g, phase);
+    factor_c<R, BY, 1, 2>(vv, cf, lds, g, phase);
+    factor_c<R, BY, 2, 2>(vv, cf, lds, g, phase);
+    factor_c<R, BY, 3, 2>(vv, cf, lds, g, phase);
+    factor_c<R, BY, 4, 2>(vv, cf, lds, g, phase);
+#pragma unroll
+    for (int k = 0; k < 2 * BY; ++k) { v1[k] = vv[0][k]; v2[k] = vv[1][k]; }
+}
+
+// v <- B(c) v / cn (one vector)
+template <typename R, int BY>
+__device__ __forceinline__ void apply_Bc(cplx<R> (&v)[2 * BY], const CoefC<R, BY>& c, cplx<R>* lds, const Geo& g, int& phase) {
+    cplx<R> vv[1][2 * BY];
+#pragma unroll
+    for (int k = 0; k < 2 * BY; ++k) vv[0][k] = v[k];
+    const CoefC<R, BY>* cf[1] = {&c};
+    factor_c<R, BY, 4, 1>(vv, cf, lds, g, phase);
+    factor_c<R, BY, 3, 1>(vv, cf, lds, g, phase);
+    factor_c<R, BY, 2, 1>(vv, cf, lds, g, phase);
+    factor_c<R, BY, 1, 1>(vv, cf, lds, g, phase);
+    factor_c<R, BY, 2, 1>(vv, cf, lds, g, phase);
+    factor_c<R, BY, 3, 1>(vv, cf, lds, g, phase);
+    factor_c<R, BY, 4, 1>(vv, cf, lds, g, phase);
+#pragma unroll
+    for (int k = 0; k < 2 * BY; ++k) v[k] = vv[0][k];
 }
 
 // Patch load / store: row r holds two adjacent sites -> one 16 B (c64) or

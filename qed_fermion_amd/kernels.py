@@ -254,6 +254,15 @@ def set_tuning(prec, by=0, t=0, tb=0):
     call("qf_set_tuning", int(prec), int(by), int(t), int(tb))
 
 
+def set_tuning_shape(prec, L, Ltau, nrhs=0, by=0, t=0, tb=0):
+    """Tiling for one lattice shape (nrhs=0: any batch size); see autotune.py."""
+    call("qf_set_tuning_shape", int(prec), int(L), int(Ltau), int(nrhs), int(by), int(t), int(tb))
+
+
+def clear_tuning():
+    call("qf_clear_tuning")
+
+
 def get_tiling(prec, nrhs, L, Ltau):
     out = (ctypes.c_int * 4)()
     call("qf_get_tiling", int(prec), nrhs, L, Ltau, out)

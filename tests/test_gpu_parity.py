@@ -233,3 +233,33 @@ def test_operator_from_hop_table(dev, prec, tol, shared):
         y = k.apply_hop(op, hop, _t(x, dev, cd), L, Lt).cpu().numpy()
         for r in range(nrhs):
             assert relerr(y[r], OPS[op](phi[0 if shared else r], x[r], L, Lt)) < tol, op
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 2e-5)])
+@pytest.mark.parametrize("L,by", [(36, 2), (40, 2), (36, 4)])
+def test_operator_repeat_multiwave(dev, prec, tol, L, by):
+    """Work-groups of 3 to 7 waves (the whole slice in one work-group) over
+    several tau-run lengths, ten launches each: every launch must match the
+    oracle.  Guards the exchange barriers and the output stores against
+    timing-dependent corruption (a store-data hazard once corrupted the last
+    four lanes of each 16-lane group only in some launches)."""
+    k = _k()
+    Lt = 40
+    print(f"[operator repeat] L={L} by={by} {prec}", flush=True)
+    phi = make_phi(L, Lt, "uniform", seed=L)
+    x = make_vec(L, Lt, seed=L + 1, nrhs=3)
+    ref = [O.apply_MdagM(phi[0], x[i], L, Lt) for i in range(3)]
+    rd, cd = (torch.float64, torch.complex128) if prec == "f64" else (torch.float32, torch.complex64)
+    pi = 1 if prec == "f64" else 0
+    pd, xd = _t(phi.reshape(1, -1), dev, rd), _t(x, dev, cd)
+    hop = k.hop_table(pd, L, Lt)
+    try:
+        for T in (1, 2, 3, 5):
+            k.set_tuning(pi, by, T, 0)
+            for _ in range(10):
+                for y in (k.apply("MdagM", pd, xd, L, Lt), k.apply_hop("MdagM", hop, xd, L, Lt)):
+                    yn = y.cpu().numpy()
+                    for i in range(3):
+                        assert relerr(yn[i], ref[i]) < tol, (T, i)
+    finally:
+        k.set_tuning(pi, 0, 0, 0)
