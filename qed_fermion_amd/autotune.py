@@ -7,8 +7,8 @@ Three shape parameters decide how a launch maps onto the MI355X:
 * ``t``   -- tau slices marched by one operator work-group (the tau pipeline
   pays one extra B application per run, so long runs are cheaper, but fewer
   work-groups fill fewer CUs);
-* ``tb``  -- tau slices per PCG update work-group (the 13-slice
-  preconditioner window re-reads 12 halo slices per work-group).
+* ``tb``  -- tau slices of a PCG update tile (16, 32 or 64; the tile's
+  13-slice preconditioner window re-reads 12 halo slices).
 
 The reference picks a fixed CUDA block (BLOCK_SIZE = (4, 8),
 hmc_sampler_batch.py:111); here the best values depend on L, Ltau, the batch
@@ -61,7 +61,7 @@ def valid_by(L, prec):
 
 
 def _t_candidates(Lt):
-    return sorted({t for t in (1, 2, 4, 6, 8, 11, 16, 22) if t <= Lt})
+    return sorted({t for t in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 14, 16, 19, 22) if t <= Lt})
 
 
 def _time(fn, reps, warm=2):
@@ -103,7 +103,7 @@ def tune_operator(prec, L, Lt, nrhs, shared_field=True, reps=10, seed=0):
     return by, t, ms, table
 
 
-def tune_pcg(prec, L, Lt, nrhs, iters=20, reps=3, seed=0, tb_list=(8, 16, 32, 64)):
+def tune_pcg(prec, L, Lt, nrhs, iters=20, reps=3, seed=0, tb_list=(16, 32, 64)):
     """Best (by, t, tb) for the batched PCG iteration (per-chain fields):
     times ``iters`` iterations of pcg_solve with rtol = 0 (no chain stops).
     Returns (by, t, tb, ms_per_iteration, table)."""
@@ -125,8 +125,6 @@ def tune_pcg(prec, L, Lt, nrhs, iters=20, reps=3, seed=0, tb_list=(8, 16, 32, 64
             table[(by, t, 0)] = _time(run, reps, warm=1) / iters
     (by, t, _), _ = min(table.items(), key=lambda kv: kv[1])
     for tb in tb_list:
-        if tb > Lt:
-            continue
         kernels.set_tuning_shape(prec, L, Lt, nrhs, by, t, tb)
         table[(by, t, tb)] = _time(run, reps, warm=1) / iters
     kernels.set_tuning_shape(prec, L, Lt, nrhs, 0, 0, 0)

@@ -73,8 +73,12 @@ inline bool by_ok(int L, int by) { return by > 0 && L % by == 0 && by % 2 == 0 &
 
 struct Tiling {
     int by, T, nA;     // operator kernels
-    int TB, nsx, nB;   // update kernel
+    int TB, nsx, nB;   // update kernel: tile of TB slices x cgu_tx(TB) sites
 };
+
+// Update-kernel tiles are 1024 threads: TB in {16, 32, 64} tau slices.
+inline bool tb_ok(int tb) { return tb == 16 || tb == 32 || tb == 64; }
+inline int cgu_tx(int tb) { return 1024 / tb; }
 
 // Tuning that applies to (prec, L, Lt, nrhs): exact shape, then any batch
 // size, then the per-precision setting.
@@ -120,12 +124,12 @@ Tiling tiling(int prec, int nrhs, int L, int Lt) {
     t.T = std::min(t.T, Lt);
     t.nA = (Lt + t.T - 1) / t.T;
     const int Vs = L * L;
-    t.nsx = (Vs + 255) / 256;
-    // tau slices per update work-group: the 13-slice preconditioner window
-    // re-reads 12 halo slices per work-group, so prefer long runs (>= 16)
-    t.TB = tu.tb ? tu.tb : (int)std::max(16LL, std::min(64LL, units * t.nsx / 1024));
-    t.TB = std::min(t.TB, Lt);
+    // update kernel tile: TB tau slices x (1024 / TB) sites; the 12-slice
+    // preconditioner halo costs 12 / TB extra reads of r and M^dag M p
+    t.TB = tu.tb ? tu.tb : (Lt <= 16 ? 16 : 32);
+    t.nsx = (Vs + cgu_tx(t.TB) - 1) / cgu_tx(t.TB);
     t.nB = t.nsx * ((Lt + t.TB - 1) / t.TB);
+    (void)units;
     return t;
 }
 
@@ -179,17 +183,26 @@ int set_lds(KernelT kern, size_t bytes) {
 inline dim3 blk_of(int L, int by) { return dim3((unsigned)(((nthreads(L, by) + 63) / 64) * 64)); }
 
 // ---------------------------------------------------------------- launchers
+// Pipelined (two vectors per exchange) where the registers allow it:
+// complex64 with 2 x 2 and 2 x 4 patches; complex128 and 2 x 6 run one vector
+// per exchange.
+template <typename R, int BY>
+constexpr bool mdm_pipe() { return sizeof(R) == 4 && BY <= 4; }
+
 template <typename R, int BY, int MODE, bool HOPT>
-int launch_mdm_k(const MdmArgs<R>& a, int nrhs, hipStream_t s) {
-    const size_t lds = lds_bytes<R>(a.L, 2);   // two vectors per exchange (tau pipeline)
-    const dim3 grid((a.Lt + a.T - 1) / a.T, nrhs), blk = blk_of(a.L, BY);
+int launch_mdm_k(const MdmArgs<R>& a0, int nrhs, hipStream_t s) {
+    MdmArgs<R> a = a0;
+    a.nrhs = nrhs;
+    constexpr bool PIPE = mdm_pipe<R, BY>();
+    const dim3 grid((unsigned)(((a.Lt + a.T - 1) / a.T) * nrhs)), blk = blk_of(a.L, BY);
+    const size_t lds = lds_bytes_c<R>(BY, (int)blk.x);
     if (blk.x <= 512) {
-        auto k = mdagm_kernel<R, BY, MODE, 512, HOPT>;
+        auto k = mdagm_kernel<R, BY, MODE, 512, HOPT, PIPE>;
         int rc = set_lds(k, lds);
         if (rc) return rc;
         hipLaunchKernelGGL(k, grid, blk, lds, s, a);
     } else if constexpr (BY == 2) {
-        auto k = mdagm_kernel<R, BY, MODE, 1024, HOPT>;
+        auto k = mdagm_kernel<R, BY, MODE, 1024, HOPT, PIPE>;
         int rc = set_lds(k, lds);
         if (rc) return rc;
         hipLaunchKernelGGL(k, grid, blk, lds, s, a);
@@ -270,15 +283,22 @@ int launch_force_by(const ForceArgs<R>& a, int nrhs, hipStream_t s) {
     return QF_OK;
 }
 
-template <typename R, int MODE>
-int launch_cgu(const CguArgs<R>& a, const Tiling& t, int nrhs, hipStream_t s) {
-    const dim3 grid(t.nsx, (a.Lt + a.TB - 1) / a.TB, nrhs);
-    if (a.ptab)
-        hipLaunchKernelGGL((cg_update_kernel<R, MODE, true>), grid, dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL((cg_update_kernel<R, MODE, false>), grid, dim3(256), 0, s, a);
+template <typename R, int MODE, bool PRE>
+int launch_cgu_p(const CguArgs<R>& a, const Tiling& t, int nrhs, hipStream_t s) {
+    const dim3 grid(t.nsx, (a.Lt + t.TB - 1) / t.TB, nrhs), blk(1024);
+    switch (t.TB) {
+        case 16: hipLaunchKernelGGL((cg_update_kernel<R, MODE, PRE, 64, 16>), grid, blk, 0, s, a); break;
+        case 32: hipLaunchKernelGGL((cg_update_kernel<R, MODE, PRE, 32, 32>), grid, blk, 0, s, a); break;
+        case 64: hipLaunchKernelGGL((cg_update_kernel<R, MODE, PRE, 16, 64>), grid, blk, 0, s, a); break;
+        default: return fail(QF_ERR_UNSUPPORTED, "update tile of %d slices", t.TB);
+    }
     QF_LAUNCHED();
     return QF_OK;
+}
+
+template <typename R, int MODE>
+int launch_cgu(const CguArgs<R>& a, const Tiling& t, int nrhs, hipStream_t s) {
+    return a.ptab ? launch_cgu_p<R, MODE, true>(a, t, nrhs, s) : launch_cgu_p<R, MODE, false>(a, t, nrhs, s);
 }
 
 inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -488,7 +508,7 @@ const char* qf_last_error(void) { return g_err.c_str(); }
 int qf_set_tuning(int prec, int by, int t, int tb) {
     if (prec != QF_PREC_F32 && prec != QF_PREC_F64) return fail(QF_ERR_INVALID, "prec %d", prec);
     if (by != 0 && (by % 2 || by < 2 || by > 6)) return fail(QF_ERR_INVALID, "patch height %d not in {0,2,4,6}", by);
-    if (t < 0 || tb < 0) return fail(QF_ERR_INVALID, "negative tile size");
+    if (t < 0 || (tb != 0 && !tb_ok(tb))) return fail(QF_ERR_INVALID, "bad tile size (t=%d, tb=%d not in {0,16,32,64})", t, tb);
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune[prec].by = by;
     g_tune[prec].t = t;
@@ -499,7 +519,7 @@ int qf_set_tuning(int prec, int by, int t, int tb) {
 int qf_set_tuning_shape(int prec, int L, int Ltau, int nrhs, int by, int t, int tb) {
     if (prec != QF_PREC_F32 && prec != QF_PREC_F64) return fail(QF_ERR_INVALID, "prec %d", prec);
     if (by != 0 && !by_ok(L, by)) return fail(QF_ERR_INVALID, "patch height %d does not fit L=%d", by, L);
-    if (t < 0 || tb < 0 || nrhs < 0 || L < 2 || Ltau < 1) return fail(QF_ERR_INVALID, "bad shape or tile size");
+    if (t < 0 || (tb != 0 && !tb_ok(tb)) || nrhs < 0 || L < 2 || Ltau < 1) return fail(QF_ERR_INVALID, "bad shape or tile size");
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (auto& e : g_shape_tune)
         if (e.prec == prec && e.L == L && e.Lt == Ltau && e.nrhs == nrhs) {
@@ -685,9 +705,8 @@ int qf_precon_vec(int prec, const void* ptab, const void* r, void* z, int nrhs, 
     if (Vs < 1 || Ltau < 1 || nrhs < 1 || nrhs > 65535) return fail(QF_ERR_INVALID, "bad sizes");
     hipStream_t s = (hipStream_t)stream;
     Tiling t{};
-    t.nsx = (Vs + 255) / 256;
-    const long long units = (long long)nrhs * Ltau;
-    t.TB = std::min<int>(Ltau, (int)std::max(2LL, std::min(64LL, units * t.nsx / 1024)));
+    t.TB = Ltau <= 16 ? 16 : 32;
+    t.nsx = (Vs + cgu_tx(t.TB) - 1) / cgu_tx(t.TB);
     auto run = [&](auto tag) -> int {
         using R = decltype(tag);
         CguArgs<R> a{};

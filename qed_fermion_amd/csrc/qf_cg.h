@@ -7,8 +7,9 @@
 // (the corners (0, Nt-r) and (Nt-r, 0) are the cyclic offsets -r / +r of rows
 // 0 and Nt-r, which the open band never uses).
 //
-// One thread owns one site and marches in tau with a 13-slice register window,
-// so every r value is read from memory about once per block instead of 13x.
+// A block owns a tile of sites x tau slices; the new residual of the tile and
+// of its 12-slice halo is staged in LDS, so every r value is read from memory
+// (1 + 12/TT) times instead of 13 times.
 //
 // Update step B_i of preconditioned_cg_fast_test (hmc_sampler_batch.py:797-844):
 //   alpha = rz_old / Re(p^dag Op);  x += alpha p;  r -= alpha Op;  z = P r;
@@ -42,26 +43,31 @@ struct CguArgs {
     const int* ru_flag;    // REFRESH: skip chains whose reliable-update flag is 0 (nullptr: never skip)
 };
 
-template <typename R, int MODE, bool PRE>
-__global__ void __launch_bounds__(256) cg_update_kernel(CguArgs<R> A) {
+// One thread per (site, slice) of a TX x TT tile; the new residual of the
+// tile's TT + 12 slices (the 13-point tau window) is staged in LDS, so no
+// thread marches in tau and all loads of a block are in flight at once.
+// Slices outside [0, Lt) of the window wrap cyclically (P is cyclic).
+// Partial sums: one (rz, rr) pair per block, blk = blockIdx.x + nsx * blockIdx.y.
+template <typename R, int MODE, bool PRE, int TX, int TT>
+__global__ void __launch_bounds__(TX * TT) cg_update_kernel(CguArgs<R> A) {
+    constexpr int H = PRE ? 12 : 0;   // halo slices
+    constexpr int ROWS = TT + H;
+    __shared__ cplx<R> ring[PRE ? ROWS * TX : 1];
     __shared__ double red[64];
     const int rho = blockIdx.z;
     const int Vs = A.Vs, Lt = A.Lt;
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = s < Vs;
-    const int t0 = blockIdx.y * A.TB;
-    const int t1 = min(t0 + A.TB, Lt);
+    const int sx = threadIdx.x % TX, st = threadIdx.x / TX;
+    const int s = blockIdx.x * TX + sx;
+    const int t0 = blockIdx.y * TT;
+    const int tau = t0 + st;
+    const bool col = s < Vs;
+    const bool own = col && tau < Lt;
+    const int sc = col ? s : 0;   // loads of padding columns read site 0 (never stored)
     const long long vo = rho * A.vs;
     const int blk = blockIdx.x + gridDim.x * blockIdx.y;
-    R alpha = R(0);
 
     if constexpr (MODE == CGU_ITER) {
-        const CgState S = A.st[rho];
-        if (S.act == 0.0) return;   // converged chain: x and r stay frozen
-        double sums[1];
-        const double* src[1] = {A.pap + (long long)rho * A.nA};
-        sum_partials<1>(src, A.nA, sums, red);
-        alpha = (R)(S.rz / sums[0]);
+        if (A.st[rho].act == 0.0) return;   // converged chain: x and r stay frozen (uniform)
     }
     if constexpr (MODE == CGU_REFRESH) {
         if (A.ru_flag != nullptr && A.ru_flag[rho] == 0) return;
@@ -73,54 +79,63 @@ __global__ void __launch_bounds__(256) cg_update_kernel(CguArgs<R> A) {
         }
     }
 
+    // ---- loads (none depends on the step size)
     const cplx<R>* rold = A.rold + vo;
     const cplx<R>* op = A.op + vo;
-    auto rn = [&](int tau) -> cplx<R> {
-        const long long i = (long long)tau * Vs + s;
-        cplx<R> r = rold[i];
-        if constexpr (MODE == CGU_ITER) {
-            const cplx<R> o = op[i];
-            r = mk<R>(r.x - alpha * o.x, r.y - alpha * o.y);
+    // window rows h = st and h = st + TT (if < ROWS): slice (t0 - H/2 + h) mod Lt
+    auto slice_of = [&](int h) { return ((t0 - H / 2 + h) % Lt + Lt) % Lt; };
+    const int ta = PRE ? slice_of(st) : min(tau, Lt - 1);
+    const bool second = PRE && st + TT < ROWS;
+    const int tb = second ? slice_of(st + TT) : ta;
+    cplx<R> ra = rold[(long long)ta * Vs + sc], rb = mk<R>(0, 0), oa = mk<R>(0, 0), ob = mk<R>(0, 0);
+    if (second) rb = rold[(long long)tb * Vs + sc];
+    if constexpr (MODE == CGU_ITER) {
+        oa = op[(long long)ta * Vs + sc];
+        if (second) ob = op[(long long)tb * Vs + sc];
+    }
+    const long long io = vo + (long long)min(tau, Lt - 1) * Vs + sc;   // own element (clamped for loads)
+    cplx<R> pp = mk<R>(0, 0), xx = mk<R>(0, 0);
+    if constexpr (MODE == CGU_ITER) {
+        pp = A.p[io];
+        xx = A.x[io];
+    }
+
+    R alpha = R(0);
+    if constexpr (MODE == CGU_ITER) {
+        double sums[1];
+        const double* src[1] = {A.pap + (long long)rho * A.nA};
+        sum_partials<1>(src, A.nA, sums, red);
+        alpha = (R)(A.st[rho].rz / sums[0]);
+        ra = mk<R>(ra.x - alpha * oa.x, ra.y - alpha * oa.y);
+        rb = mk<R>(rb.x - alpha * ob.x, rb.y - alpha * ob.y);
+    }
+
+    cplx<R> r, zz;
+    if constexpr (PRE) {
+        ring[st * TX + sx] = ra;
+        if (second) ring[(st + TT) * TX + sx] = rb;
+        __syncthreads();
+        r = ring[(st + H / 2) * TX + sx];
+        const R* c = A.ptab + min(tau, Lt - 1) * 13;
+        zz = mk<R>(0, 0);
+#pragma unroll
+        for (int k = 0; k < 13; ++k) {
+            const cplx<R> w = ring[(st + k) * TX + sx];
+            zz = mk<R>(zz.x + c[k] * w.x, zz.y + c[k] * w.y);
         }
-        return r;
-    };
+    } else {
+        r = ra;
+        zz = r;
+    }
 
     double rr = 0.0, rz = 0.0;
-    cplx<R> w[13];
-#pragma unroll
-    for (int k = 0; k < 13; ++k) w[k] = mk<R>(0, 0);
-    if (valid) {
-        if constexpr (PRE) {
-#pragma unroll
-            for (int k = 1; k < 13; ++k) w[k] = rn(((t0 - 7 + k) % Lt + Lt) % Lt);
-        }
-        for (int tau = t0; tau < t1; ++tau) {
-            cplx<R> r, zz;
-            if constexpr (PRE) {
-#pragma unroll
-                for (int k = 0; k < 12; ++k) w[k] = w[k + 1];
-                w[12] = rn((tau + 6) % Lt);
-                const R* c = A.ptab + tau * 13;
-                zz = mk<R>(0, 0);
-#pragma unroll
-                for (int k = 0; k < 13; ++k) zz = mk<R>(zz.x + c[k] * w[k].x, zz.y + c[k] * w[k].y);
-                r = w[6];
-            } else {
-                r = rn(tau);
-                zz = r;
-            }
-            const long long i = vo + (long long)tau * Vs + s;
-            if constexpr (MODE == CGU_ITER) {
-                const cplx<R> pp = A.p[i];
-                cplx<R> xx = A.x[i];
-                A.x[i] = mk<R>(xx.x + alpha * pp.x, xx.y + alpha * pp.y);
-            }
-            if constexpr (MODE == CGU_INIT) A.x[i] = mk<R>(0, 0);
-            if constexpr (MODE == CGU_INIT || MODE == CGU_ITER) A.rnew[i] = r;
-            A.z[i] = zz;
-            rr += (double)r.x * r.x + (double)r.y * r.y;
-            rz += (double)r.x * zz.x + (double)r.y * zz.y;
-        }
+    if (own) {
+        if constexpr (MODE == CGU_ITER) A.x[io] = mk<R>(xx.x + alpha * pp.x, xx.y + alpha * pp.y);
+        if constexpr (MODE == CGU_INIT) A.x[io] = mk<R>(0, 0);
+        if constexpr (MODE == CGU_INIT || MODE == CGU_ITER) A.rnew[io] = r;
+        A.z[io] = zz;
+        rr = (double)r.x * r.x + (double)r.y * r.y;
+        rz = (double)r.x * zz.x + (double)r.y * zz.y;
     }
     if constexpr (MODE != CGU_PRECON) {
         double v[2] = {rz, rr};

@@ -46,6 +46,7 @@ struct MdmArgs {
     cplx<R>* y;
     long long vs;
     int L, Lt, T;
+    int nrhs;               // set by the launcher (grid = runs * nrhs)
     Hop<R> hp;
     double* dot_part;       // [nrhs][nblk]: ||(Mx)_tau||^2 = Re(x^dag M^dag M x) over the run (optional)
     // MODE 1 (CG step A): x_tau := z_tau + beta p_tau, written to pnew
@@ -99,18 +100,28 @@ __device__ __forceinline__ void touch_lines(unsigned (&d)[NTOUCH], __amdgpu_buff
     }
 }
 
-template <typename R, int BY, int MODE, int NT, bool HOPT>
+// PIPE = false: one vector per exchange (14 barriers per slice) and only two
+// coefficient sets live, for patches whose pipelined form exceeds the
+// register file (2 x 6 patches, complex128):
+//   U = B_tau x_tau;  y_{tau+1} = x_{tau+1} + sM U;  W = B_tau y_{tau+1};
+//   out_tau = y_tau + sD W.
+template <typename R, int BY, int MODE, int NT, bool HOPT, bool PIPE>
 __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
     extern __shared__ __align__(16) unsigned char qf_smem[];
     cplx<R>* lds = reinterpret_cast<cplx<R>*>(qf_smem);
     __shared__ double red[64];
     constexpr unsigned S = sizeof(cplx<R>);
-    const int rho = blockIdx.y;
+    // 1-D grid, right-hand side fastest: the work-groups dispatched together
+    // (round-robin over the XCDs) march the same tau run, so a shared field's
+    // coefficients are fetched into each XCD's L2 once for all of them.
+    const int rho = blockIdx.x % A.nrhs;
+    const int run = blockIdx.x / A.nrhs;
     const int L = A.L, Lt = A.Lt, Vs = L * L;
-    const int t0 = blockIdx.x * A.T;
+    const int t0 = run * A.T;
     const int t1 = min(t0 + A.T, Lt);
     const Geo g = make_geo<BY>(L, threadIdx.x);
     const PatchOff po = patch_off<R>(g);
+    const XAddr xa = make_xaddr<BY>(g);
     const unsigned vbytes = (unsigned)Vs * Lt * S;   // one vector
     const unsigned sstride = (unsigned)Vs * S;       // one slice of a vector
     if constexpr (MODE == MDM_RESID) {
@@ -178,7 +189,7 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
                 if (act != 0.0) it += 1.0;
             }
         }
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (run == 0 && threadIdx.x == 0) {
             CgState o; o.rz = rzv; o.act = act; o.err = err; o.iters = it;
             A.st_out[rho] = o;
             if (A.iter == 0) A.normb[rho] = nb;
@@ -196,7 +207,96 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
     const R cn = A.hp.cn;   // apply_B returns B v / cn
     cplx<R> X[2 * BY], Y[2 * BY], Yn[2 * BY], U[2 * BY], W[2 * BY];
     double dsum = 0.0;
+    // out_tau = y_tau + sD W (MODE_RESID: r = b - out in complex64)
+    auto emit = [&](int tau, const cplx<R> (&bb)[2 * BY], const cplx<R> (&Wv)[2 * BY]) {
+        cplx<R> Z[2 * BY];
+        axpy_patch<R, BY>(Z, Y, sgnD(tau, Lt) * cn, Wv);
+        if (g.valid) {
+            if constexpr (MODE == MDM_RESID) {
+#pragma unroll
+                for (int k = 0; k < 2 * BY; ++k) {
+                    const R rx_ = bb[k].x - Z[k].x, ry_ = bb[k].y - Z[k].y;
+                    dsum += (double)rx_ * rx_ + (double)ry_ * ry_;
+                    Z[k] = mk<R>(rx_, ry_);
+                }
+                float2* rp = r_lo + (long long)tau * Vs + g.y0 * L + 2 * g.a;
+#pragma unroll
+                for (int r = 0; r < BY; ++r)
+                    *reinterpret_cast<float4*>(rp + r * L) =
+                        make_float4((float)Z[2 * r].x, (float)Z[2 * r].y, (float)Z[2 * r + 1].x, (float)Z[2 * r + 1].y);
+            } else {
+                store_patch_g<R, BY>(Z, yout + (long long)tau * Vs, po, L);
+                dsum += redot_patch<R, BY>(Y, Y);
+            }
+        }
+    };
 
+    constexpr bool REGPF = PIPE && mdm_regpf<R, BY, MODE>();
+    constexpr int NTOUCH = (BY * (int)S + 31) / 32;   // 128-byte lines of a hop slice per lane
+    unsigned tacc = 0, td[NTOUCH];
+    // coefficient source of one slice: hop entries or phi
+    const unsigned cslice = HOPT ? 2u * sstride : 2u * (unsigned)Vs * (unsigned)sizeof(R), nlanes = blockDim.x;
+    const __amdgpu_buffer_rsrc_t rs_c = HOPT ? rs_h : buf_rsrc(phi, cslice * (unsigned)Lt);
+    if constexpr (!PIPE) {
+        // x_{tau+1} is not kept across the second B: it is read again (an L2
+        // hit: MODE_CG from the p just stored, else from x) while W is being
+        // transformed, and x_{tau+2} is requested at the start of the next step.
+        const __amdgpu_buffer_rsrc_t rs_re = MODE == MDM_CG ? buf_rsrc(A.pnew + vo, vbytes) : rs_a;
+        // 2 x 6 patches hold a single coefficient set: C_tau is loaded at the
+        // start of step tau (its lines were touched one step earlier)
+        constexpr bool ONESET = BY >= 6;
+        make(U, rw);   // x_{t0-1}
+        make(X, rx);   // x_{t0}
+        if constexpr (MODE == MDM_CG) {
+            if (g.valid) store_patch_g<R, BY>(X, pnew + (long long)t0 * Vs, po, L);
+        }
+        touch_lines<NTOUCH>(td, rs_c, (unsigned)t0n * cslice, cslice, nlanes);
+        apply_Bc<R, BY>(U, c2, lds, xa, phase);                 // B_{t0-1} x_{t0-1}
+        axpy_patch<R, BY>(Y, X, sgnM(t0) * cn, U);              // y_{t0}
+        // step tau: X = x_tau on entry; cA = C_tau (loaded), cB receives C_{tau+1}
+        auto sstep = [&](int tau, CoefC<R, BY>& cA, CoefC<R, BY>& cB) {
+            const bool more = tau + 1 < t1;
+            const int tn = (tau + 1 == Lt) ? 0 : tau + 1;
+            const int tnn = (tn + 1 == Lt) ? 0 : tn + 1;
+            cplx<R> bb[2 * BY];
+            if constexpr (MODE == MDM_RESID) load_patch_buf<R, BY>(bb, rs_b, po, (unsigned)tau * sstride, L);
+            ldraw(rx, tn);                                      // x_{tau+1} (the halo slice when !more)
+            if constexpr (ONESET) {
+                if (tau != t0) coef(cA, tau);                   // touched one step ahead: an L2 hit
+            }
+            if (more) {
+                if constexpr (!ONESET) coef(cB, tn);
+#pragma unroll
+                for (int k = 0; k < NTOUCH; ++k) tacc ^= td[k];
+                if (tau + 2 < t1) touch_lines<NTOUCH>(td, rs_c, (unsigned)tnn * cslice, cslice, nlanes);
+            }
+#pragma unroll
+            for (int k = 0; k < 2 * BY; ++k) U[k] = X[k];
+            apply_Bc<R, BY>(U, cA, lds, xa, phase);             // B_tau x_tau
+            make(X, rx);                                        // x_{tau+1}
+            if constexpr (MODE == MDM_CG) {
+                if (more && g.valid) store_patch_g<R, BY>(X, pnew + (long long)tn * Vs, po, L);
+            }
+            axpy_patch<R, BY>(W, X, sgnM(tn) * cn, U);          // y_{tau+1}
+#pragma unroll
+            for (int k = 0; k < 2 * BY; ++k) Yn[k] = W[k];
+            if (more) load_patch_buf<R, BY>(X, rs_re, po, (unsigned)tn * sstride, L);   // x_{tau+1} again
+            apply_Bc<R, BY>(W, cA, lds, xa, phase);             // B_tau y_{tau+1}
+            emit(tau, bb, W);
+#pragma unroll
+            for (int k = 0; k < 2 * BY; ++k) Y[k] = Yn[k];
+        };
+        for (int tau = t0;;) {
+            sstep(tau, c0, c1);
+            if (++tau >= t1) break;
+            if constexpr (ONESET) {
+                sstep(tau, c0, c1);
+            } else {
+                sstep(tau, c1, c0);
+            }
+            if (++tau >= t1) break;
+        }
+    } else {
     make(U, rw);   // x_{t0-1}
     make(X, rx);   // x_{t0}
     if constexpr (MODE == MDM_CG) {
@@ -204,19 +304,13 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
     }
 #pragma unroll
     for (int k = 0; k < 2 * BY; ++k) W[k] = X[k];
-    constexpr bool REGPF = mdm_regpf<R, BY, MODE>();
-    constexpr int NTOUCH = (BY * (int)S + 31) / 32;   // 128-byte lines of a hop slice per lane
-    unsigned tacc = 0, td[NTOUCH];
-    // coefficient source of one slice: hop entries or phi
-    const unsigned cslice = HOPT ? 2u * sstride : 2u * (unsigned)Vs * (unsigned)sizeof(R), nlanes = blockDim.x;
-    const __amdgpu_buffer_rsrc_t rs_c = HOPT ? rs_h : buf_rsrc(phi, cslice * (unsigned)Lt);
     if constexpr (REGPF) {
         if (t0 + 1 < t1) coef(c1, t0 + 1);
     } else {
         touch_lines<NTOUCH>(td, rs_c, (unsigned)t0n * cslice, cslice, nlanes);
     }
     ldraw(rx, t0n);   // x_{t0+1}
-    apply_B2c<R, BY>(U, c2, W, c0, lds, g, phase);
+    apply_B2c<R, BY>(U, c2, W, c0, lds, xa, phase);
     axpy_patch<R, BY>(Y, X, sgnM(t0) * cn, U);
     make(X, rx);
     axpy_patch<R, BY>(Yn, X, sgnM(t0n) * cn, W);
@@ -245,30 +339,11 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
                 if (tau + 2 < t1) touch_lines<NTOUCH>(td, rs_c, (unsigned)tnn * cslice, cslice, nlanes);
             }
             ldraw(rx, tnn);
-            apply_B2c<R, BY>(W, cA, U, cB, lds, g, phase);   // W = B_tau y_{tau+1}, U = B_{tau+1} x_{tau+1}
+            apply_B2c<R, BY>(W, cA, U, cB, lds, xa, phase);   // W = B_tau y_{tau+1}, U = B_{tau+1} x_{tau+1}
         } else {
-            apply_Bc<R, BY>(W, cA, lds, g, phase);
+            apply_Bc<R, BY>(W, cA, lds, xa, phase);
         }
-        cplx<R> Z[2 * BY];
-        axpy_patch<R, BY>(Z, Y, sgnD(tau, Lt) * cn, W);
-        if (g.valid) {
-            if constexpr (MODE == MDM_RESID) {
-#pragma unroll
-                for (int k = 0; k < 2 * BY; ++k) {
-                    const R rx_ = bb[k].x - Z[k].x, ry_ = bb[k].y - Z[k].y;
-                    dsum += (double)rx_ * rx_ + (double)ry_ * ry_;
-                    Z[k] = mk<R>(rx_, ry_);
-                }
-                float2* rp = r_lo + (long long)tau * Vs + g.y0 * L + 2 * g.a;
-#pragma unroll
-                for (int r = 0; r < BY; ++r)
-                    *reinterpret_cast<float4*>(rp + r * L) =
-                        make_float4((float)Z[2 * r].x, (float)Z[2 * r].y, (float)Z[2 * r + 1].x, (float)Z[2 * r + 1].y);
-            } else {
-                store_patch_g<R, BY>(Z, yout + (long long)tau * Vs, po, L);
-                dsum += redot_patch<R, BY>(Y, Y);
-            }
-        }
+        emit(tau, bb, W);
         if (more) {
 #pragma unroll
             for (int k = 0; k < 2 * BY; ++k) Y[k] = Yn[k];
@@ -285,6 +360,7 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
         if (++tau >= t1) break;
     }
 
+    }
     if constexpr (!REGPF) {
         // fold the touched dwords in; T < 0 never holds, so this only keeps the loads alive
 #pragma unroll
@@ -294,7 +370,7 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
     if (A.dot_part != nullptr) {
         double v[1] = {dsum};
         block_sum<1>(v, red);
-        if (threadIdx.x == 0) A.dot_part[(long long)rho * gridDim.x + blockIdx.x] = v[0];
+        if (threadIdx.x == 0) A.dot_part[(long long)rho * (gridDim.x / A.nrhs) + run] = v[0];
     }
 }
 

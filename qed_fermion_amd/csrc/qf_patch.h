@@ -458,28 +458,57 @@ template <typename R> __device__ __forceinline__ cplx<R> cjmul_h(cplx<R> h, cplx
     return mk<R>(fma(h.x, u.x, h.y * u.y), fma(h.x, u.y, -h.y * u.x));
 }
 
+// Lane-indexed exchange layout of the compact factors: a phase buffer holds
+// NPL = max(BY, 2) planes of nl slots (nl = lanes of the work-group), slot =
+// patch index (= threadIdx.x for valid lanes), NV complex per slot.  A
+// wave's writes are then one contiguous run and its reads a run shifted by
+// one patch (or one patch row), so LDS accesses are free of bank conflicts.
+//   x-group, j-th edge row: plane 2j = EL (cx=0 site, raw),
+//                           plane 2j+1 = ER (cx=1 site, premultiplied)
+//   y-group: plane 0 = BOT (r=0 site, raw), plane 1 = TOP (r=BY-1, premultiplied)
+// Both phase buffers are sized for NV = 2 (lds_bytes_c), 2*NPL*nl slots apart.
+__host__ __device__ constexpr int npl_of(int BY) { return BY > 2 ? BY : 2; }
+template <typename R>
+inline size_t lds_bytes_c(int BY, int nlanes) { return (size_t)2 * npl_of(BY) * nlanes * 2 * sizeof(cplx<R>); }
+
+struct XAddr {
+    int me;            // own slot
+    int l, r, d, u;    // slots of the left / right / lower / upper neighbour patches
+    int nl;            // slots per plane
+};
+
+template <int BY>
+__device__ __forceinline__ XAddr make_xaddr(const Geo& g) {
+    const int na = g.L >> 1;
+    XAddr x;
+    x.me = threadIdx.x;
+    x.nl = blockDim.x;
+    const int am = g.a == 0 ? na - 1 : g.a - 1, ap = g.a + 1 == na ? 0 : g.a + 1;
+    const int bm = g.b == 0 ? g.nb - 1 : g.b - 1, bp = g.b + 1 == g.nb ? 0 : g.b + 1;
+    x.l = g.b * na + am;
+    x.r = g.b * na + ap;
+    x.d = bm * na + g.a;
+    x.u = bp * na + g.a;
+    return x;
+}
+
 // One normalised factor (I + th K_G) on NV vectors with coefficient sets
-// cf[n], one exchange barrier for all of them.  The two alternating buffers
-// are 2 * xstride(L) slots apart whatever NV is (LDS sized lds_bytes(L, 2)),
-// so factors on one and on two vectors can follow each other without a
-// write-after-read overlap.
+// cf[n], one exchange barrier for all of them.  Inside pairs are updated
+// between the edge writes and the barrier.
 template <typename R, int BY, int G, int NV>
-__device__ __forceinline__ void factor_c(cplx<R> (*v)[2 * BY], const CoefC<R, BY>* const* cf, cplx<R>* lds, const Geo& g, int& phase) {
+__device__ __forceinline__ void factor_c(cplx<R> (*v)[2 * BY], const CoefC<R, BY>* const* cf, cplx<R>* lds, const XAddr& xa, int& phase) {
     static_assert(NV == 1 || NV == 2, "one or two vectors per exchange");
-    const int L = g.L;
-    const int na = L >> 1;
-    const int plane = xstride(L) >> 1;
-    cplx<R>* buf = lds + (phase & 1) * 2 * xstride(L);
+    constexpr int NPL = npl_of(BY);
+    cplx<R>* buf = lds + (phase & 1) * (NPL * 2) * xa.nl;
+    auto at = [&](int plane, int slot) { return buf + (plane * xa.nl + slot) * NV; };
     using IO = SlotIO<R, NV>;
     if constexpr (G == 1 || G == 3) {
         constexpr int rc = (G == 1) ? 1 : 0;   // edge rows have r % 2 == rc
 #pragma unroll
-        for (int r = rc; r < BY; r += 2) {
-            const int o = ((g.y0 + r) >> 1) * na + g.a;
-            IO::put(buf + o * NV, [&](int n) { return v[n][2 * r]; });                                   // EL: raw
-            IO::put(buf + (plane + o) * NV, [&](int n) { return cjmul_h<R>(cf[n]->ex[r][1], v[n][2 * r + 1]); });   // ER: premultiplied
+        for (int r = rc, j = 0; r < BY; r += 2, ++j) {
+            IO::put(at(2 * j, xa.me), [&](int n) { return v[n][2 * r]; });
+            IO::put(at(2 * j + 1, xa.me), [&](int n) { return cjmul_h<R>(cf[n]->ex[r][1], v[n][2 * r + 1]); });
         }
-        // inside pairs meanwhile
 #pragma unroll
         for (int n = 0; n < NV; ++n)
 #pragma unroll
@@ -489,20 +518,16 @@ __device__ __forceinline__ void factor_c(cplx<R> (*v)[2 * BY], const CoefC<R, BY
                 v[n][2 * r + 1] = pair_addc<R>(v1, h, v0);
             }
         lds_barrier();
-        const int am = g.a == 0 ? na - 1 : g.a - 1, ap = g.a + 1 == na ? 0 : g.a + 1;
 #pragma unroll
-        for (int r = rc; r < BY; r += 2) {
-            const int rowo = ((g.y0 + r) >> 1) * na;
-            IO::get(buf + (plane + rowo + am) * NV, [&](int n, cplx<R> p) { v[n][2 * r] = cadd<R>(v[n][2 * r], p); });
-            IO::get(buf + (rowo + ap) * NV, [&](int n, cplx<R> w) { v[n][2 * r + 1] = pair_add<R>(v[n][2 * r + 1], cf[n]->ex[r][1], w); });
+        for (int r = rc, j = 0; r < BY; r += 2, ++j) {
+            IO::get(at(2 * j + 1, xa.l), [&](int n, cplx<R> p) { v[n][2 * r] = cadd<R>(v[n][2 * r], p); });
+            IO::get(at(2 * j, xa.r), [&](int n, cplx<R> w) { v[n][2 * r + 1] = pair_add<R>(v[n][2 * r + 1], cf[n]->ex[r][1], w); });
         }
     } else {
         constexpr int cc = (G == 2) ? 1 : 0;   // edge column
         constexpr int ci = 1 - cc;             // inside column
-        const int yplane = g.nb * na;
-        const int o = g.b * na + g.a;
-        IO::put(buf + o * NV, [&](int n) { return v[n][cc]; });                                                         // BOT: raw
-        IO::put(buf + (yplane + o) * NV, [&](int n) { return cjmul_h<R>(cf[n]->ey[cc][BY - 1], v[n][2 * (BY - 1) + cc]); });   // TOP
+        IO::put(at(0, xa.me), [&](int n) { return v[n][cc]; });
+        IO::put(at(1, xa.me), [&](int n) { return cjmul_h<R>(cf[n]->ey[cc][BY - 1], v[n][2 * (BY - 1) + cc]); });
 #pragma unroll
         for (int n = 0; n < NV; ++n) {
 #pragma unroll
@@ -519,9 +544,8 @@ __device__ __forceinline__ void factor_c(cplx<R> (*v)[2 * BY], const CoefC<R, BY
             }
         }
         lds_barrier();
-        const int bm = g.b == 0 ? g.nb - 1 : g.b - 1, bp = g.b + 1 == g.nb ? 0 : g.b + 1;
-        IO::get(buf + (yplane + bm * na + g.a) * NV, [&](int n, cplx<R> p) { v[n][cc] = cadd<R>(v[n][cc], p); });
-        IO::get(buf + (bp * na + g.a) * NV, [&](int n, cplx<R> w) {
+        IO::get(at(1, xa.d), [&](int n, cplx<R> p) { v[n][cc] = cadd<R>(v[n][cc], p); });
+        IO::get(at(0, xa.u), [&](int n, cplx<R> w) {
             v[n][2 * (BY - 1) + cc] = pair_add<R>(v[n][2 * (BY - 1) + cc], cf[n]->ey[cc][BY - 1], w);
         });
     }
@@ -531,7 +555,7 @@ __device__ __forceinline__ void factor_c(cplx<R> (*v)[2 * BY], const CoefC<R, BY
 // v1 <- B(c1) v1 / cn and v2 <- B(c2) v2 / cn with 7 barriers.
 template <typename R, int BY>
 __device__ __forceinline__ void apply_B2c(cplx<R> (&v1)[2 * BY], const CoefC<R, BY>& c1, cplx<R> (&v2)[2 * BY], const CoefC<R, BY>& c2,
-                                          cplx<R>* lds, const Geo& g, int& phase) {
+                                          cplx<R>* lds, const XAddr& g, int& phase) {
     cplx<R> vv[2][2 * BY];
 #pragma unroll
     for (int k = 0; k < 2 * BY; ++k) { vv[0][k] = v1[k]; vv[1][k] = v2[k]; }
@@ -549,7 +573,7 @@ __device__ __forceinline__ void apply_B2c(cplx<R> (&v1)[2 * BY], const CoefC<R, 
 
 // v <- B(c) v / cn (one vector)
 template <typename R, int BY>
-__device__ __forceinline__ void apply_Bc(cplx<R> (&v)[2 * BY], const CoefC<R, BY>& c, cplx<R>* lds, const Geo& g, int& phase) {
+__device__ __forceinline__ void apply_Bc(cplx<R> (&v)[2 * BY], const CoefC<R, BY>& c, cplx<R>* lds, const XAddr& g, int& phase) {
     cplx<R> vv[1][2 * BY];
 #pragma unroll
     for (int k = 0; k < 2 * BY; ++k) vv[0][k] = v[k];
