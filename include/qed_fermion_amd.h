@@ -123,6 +123,28 @@ int qf_pcg_mixed_solve(const double* phi64, const float* phi32, long long phi_cs
                        double dtau, double t, const float* ptab32, double rtol, int max_iter, int ru_period, void* work,
                        size_t work_bytes, double* err_out, double* iters_out, void* stream);
 
+/* Ragged batch: nprob independent lattices of different L (one chain each)
+ * in the same launches -- the batching of several lattices of uneven size
+ * that BASELINE's batching configuration asks for; no counterpart in the
+ * reference, whose batches share one L.  Data are concatenated per lattice
+ * (vectors: Ltau*L*L complex each; phi: 2*Ltau*L*L reals; preconditioner
+ * tables: 13*Ltau each).  qf_ragged_plan fills a host plan of
+ * qf_ragged_plan_bytes(nprob) bytes; the caller keeps an identical copy in
+ * device memory and passes both (the host copy sizes the launches, the device
+ * copy is read by the kernels).  Every lattice runs with 2 x 2 patches. */
+size_t qf_ragged_plan_bytes(int nprob);
+int qf_ragged_plan(int nprob, const int* L, const int* Ltau, void* plan, size_t plan_bytes);
+/* y = M^dag M x for every lattice of the batch in one launch (op must be QF_OP_MDAGM). */
+int qf_apply_ragged(int op, int prec, const void* plan_host, const void* plan_dev, const void* phi, const void* x, void* y, double dtau,
+                    double t, void* stream);
+/* Batched PCG over a ragged batch (the recurrence of qf_pcg_solve; err_out and
+ * iters_out per lattice); every iteration is one operator and one update launch
+ * for the whole batch. */
+size_t qf_pcg_ragged_workspace_bytes(int prec, const void* plan_host);
+int qf_pcg_solve_ragged(int prec, const void* plan_host, const void* plan_dev, const void* phi, const void* b, void* x, double dtau,
+                        double t, const void* ptab, double rtol, int max_iter, void* work, size_t work_bytes, double* err_out,
+                        double* iters_out, void* stream);
+
 /* Fermion force Ft = -dS_f/dphi given xi = (M^dag M)^{-1} psi: the second half of
  * force_f_fast (hmc_sampler_batch.py:2174-2227, per-tau _C.b_vec_per_tau calls and
  * df_dot gathers 1969-2006) fused into one launch.  Ft has the phi layout. */

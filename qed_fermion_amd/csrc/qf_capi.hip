@@ -301,6 +301,40 @@ int launch_cgu(const CguArgs<R>& a, const Tiling& t, int nrhs, hipStream_t s) {
     return a.ptab ? launch_cgu_p<R, MODE, true>(a, t, nrhs, s) : launch_cgu_p<R, MODE, false>(a, t, nrhs, s);
 }
 
+// ------------------------------------------------------------ ragged batch
+// Every lattice of the batch runs with 2 x 2 patches (valid for every even L)
+// in one work-group size (the largest lattice's); smaller lattices leave the
+// surplus lanes as padding.
+template <typename R, int MODE, bool HOPT>
+int launch_mdm_ragged(const MdmArgs<R>& a, const RaggedHead& h, hipStream_t s) {
+    const dim3 grid((unsigned)h.nwg_op), blk((unsigned)h.nthreads);
+    const size_t lds = lds_bytes_c<R>(2, h.nthreads);
+    if (h.nthreads <= 512) {
+        auto k = mdagm_kernel<R, 2, MODE, 512, HOPT, mdm_pipe<R, 2>()>;
+        int rc = set_lds(k, lds);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k, grid, blk, lds, s, a);
+    } else {
+        auto k = mdagm_kernel<R, 2, MODE, 1024, HOPT, mdm_pipe<R, 2>()>;
+        int rc = set_lds(k, lds);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k, grid, blk, lds, s, a);
+    }
+    QF_LAUNCHED();
+    return QF_OK;
+}
+
+template <typename R, int MODE>
+int launch_cgu_ragged(const CguArgs<R>& a, const RaggedHead& h, hipStream_t s) {
+    const dim3 grid((unsigned)h.nwg_upd), blk(1024);
+    if (a.ptab)
+        hipLaunchKernelGGL((cg_update_kernel<R, MODE, true, 32, 32>), grid, blk, 0, s, a);
+    else
+        hipLaunchKernelGGL((cg_update_kernel<R, MODE, false, 32, 32>), grid, blk, 0, s, a);
+    QF_LAUNCHED();
+    return QF_OK;
+}
+
 inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 template <typename R>
@@ -457,10 +491,12 @@ int pcg_iterate(const R* phi, long long phi_cs, const cplx<R>* b_in, cplx<R>* x,
         rc = reliable_update(max_iter, last, 1, last);
         if (rc) return rc;
         hipLaunchKernelGGL(cg_finalize_kernel, dim3(nrhs), dim3(64), 0, s, (const double*)w.rp64, t64->nA,
-                           (const CgState*)(w.st + last * nrhs), (const double*)w.normb, rtol, 1, err_out, iters_out);
+                           (const CgState*)(w.st + last * nrhs), (const double*)w.normb, rtol, 1, err_out, iters_out,
+                           (const Prob*)nullptr);
     } else {
         hipLaunchKernelGGL(cg_finalize_kernel, dim3(nrhs), dim3(64), 0, s, (const double*)(w.rrp + last * ps), t.nB,
-                           (const CgState*)(w.st + last * nrhs), (const double*)w.normb, rtol, 0, err_out, iters_out);
+                           (const CgState*)(w.st + last * nrhs), (const double*)w.normb, rtol, 0, err_out, iters_out,
+                           (const Prob*)nullptr);
     }
     QF_LAUNCHED();
     return QF_OK;
@@ -772,6 +808,158 @@ int qf_pcg_mixed_solve(const double* phi64, const float* phi32, long long phi_cs
     return pcg_iterate<float>(phi32, phi_cs, (const float2*)w.BLO, (float2*)w.XLO, nrhs, L, Ltau, make_hop<float>(dtau, t), ptab32, rtol,
                               max_iter, w, tl, s, ru_period, phi64, phi_cs, (const double2*)b, (double2*)x, &hp64, &t64, err_out,
                               iters_out);
+}
+
+// ---------------------------------------------------------------- ragged batch
+size_t qf_ragged_plan_bytes(int nprob) {
+    if (nprob < 1) return 0;
+    return sizeof(RaggedHead) + (size_t)nprob * sizeof(Prob);
+}
+
+int qf_ragged_plan(int nprob, const int* Ls, const int* Lts, void* plan, size_t plan_bytes) {
+    if (nprob < 1 || nprob > 4096 || !Ls || !Lts || !plan) return fail(QF_ERR_INVALID, "bad ragged plan arguments");
+    if (plan_bytes < qf_ragged_plan_bytes(nprob)) return fail(QF_ERR_INVALID, "plan buffer of %zu bytes, need %zu", plan_bytes,
+                                                               qf_ragged_plan_bytes(nprob));
+    RaggedHead* h = static_cast<RaggedHead*>(plan);
+    Prob* P = reinterpret_cast<Prob*>(h + 1);
+    std::memset(plan, 0, qf_ragged_plan_bytes(nprob));
+    h->nprob = nprob; h->by = 2; h->tb = 32;
+    int maxthr = 0;
+    long long voff = 0, toff = 0, wg = 0, bwg = 0, pa = 0, pb = 0;
+    for (int k = 0; k < nprob; ++k) {
+        const int L = Ls[k], Lt = Lts[k];
+        if (L < 2 || (L & 1)) return fail(QF_ERR_ODD_L, "lattice %d: L=%d must be even and >= 2", k, L);
+        if (!by_ok(L, 2)) return fail(QF_ERR_UNSUPPORTED, "lattice %d: L=%d exceeds the 1024-thread work-group of 2x2 patches", k, L);
+        if (Lt < 1) return fail(QF_ERR_INVALID, "lattice %d: Ltau=%d", k, Lt);
+        Prob& q = P[k];
+        q.L = L; q.Lt = Lt;
+        q.T = std::min(Lt, 4);
+        q.nA = (Lt + q.T - 1) / q.T;
+        q.wg0 = (int)wg;
+        q.nsx = (L * L + cgu_tx(h->tb) - 1) / cgu_tx(h->tb);
+        q.nB = q.nsx * ((Lt + h->tb - 1) / h->tb);
+        q.bwg0 = (int)bwg;
+        q.voff = voff; q.poff = 2 * voff; q.hoff = 2 * voff; q.toff = toff;
+        q.pa0 = (int)pa; q.pb0 = (int)pb;
+        maxthr = std::max(maxthr, nthreads(L, 2));
+        wg += q.nA; bwg += q.nB; pa += q.nA; pb += q.nB;
+        voff += (long long)L * L * Lt; toff += 13LL * Lt;
+        if (wg > (1LL << 30) || bwg > (1LL << 30)) return fail(QF_ERR_INVALID, "ragged batch too large");
+    }
+    h->nthreads = ((maxthr + 63) / 64) * 64;
+    h->nwg_op = (int)wg; h->nwg_upd = (int)bwg;
+    h->nA_tot = (int)pa; h->nB_tot = (int)pb;
+    h->nsites = voff; h->ntab = toff;
+    return QF_OK;
+}
+
+static int ragged_head(const void* plan_host, const void* plan_dev, const RaggedHead** h, const Prob** pd) {
+    if (!plan_host || !plan_dev) return fail(QF_ERR_INVALID, "null plan");
+    *h = static_cast<const RaggedHead*>(plan_host);
+    if ((*h)->nprob < 1 || (*h)->by != 2 || (*h)->nthreads < 64 || (*h)->nthreads > 1024)
+        return fail(QF_ERR_INVALID, "not a ragged plan (build it with qf_ragged_plan)");
+    *pd = reinterpret_cast<const Prob*>(static_cast<const RaggedHead*>(plan_dev) + 1);
+    return QF_OK;
+}
+
+int qf_apply_ragged(int op, int prec, const void* plan_host, const void* plan_dev, const void* phi, const void* x, void* y, double dtau,
+                    double t, void* stream) {
+    const RaggedHead* h;
+    const Prob* pd;
+    int rc = ragged_head(plan_host, plan_dev, &h, &pd);
+    if (rc) return rc;
+    if (op != QF_OP_MDAGM) return fail(QF_ERR_UNSUPPORTED, "ragged batches apply M^dag M only (op %d)", op);
+    if (!phi || !x || !y) return fail(QF_ERR_INVALID, "null pointer");
+    if (x == y) return fail(QF_ERR_INVALID, "x and y must not alias");
+    hipStream_t s = (hipStream_t)stream;
+    auto run = [&](auto tag) -> int {
+        using R = decltype(tag);
+        MdmArgs<R> a{};
+        a.phi = (const R*)phi; a.x = (const cplx<R>*)x; a.y = (cplx<R>*)y; a.hp = make_hop<R>(dtau, t);
+        a.prob = pd; a.nprob = h->nprob; a.nrhs = 1;
+        return launch_mdm_ragged<R, MDM_PLAIN, false>(a, *h, s);
+    };
+    if (prec == QF_PREC_F32) return run(float{});
+    if (prec == QF_PREC_F64) return run(double{});
+    return fail(QF_ERR_INVALID, "prec %d", prec);
+}
+
+size_t qf_pcg_ragged_workspace_bytes(int prec, const void* plan_host) {
+    if (!plan_host || (prec != QF_PREC_F32 && prec != QF_PREC_F64)) return 0;
+    const RaggedHead* h = static_cast<const RaggedHead*>(plan_host);
+    if (h->nprob < 1) return 0;
+    const size_t vb = al256((size_t)h->nsites * (prec == QF_PREC_F64 ? 16 : 8));
+    return 6 * vb + 2 * vb + al256(sizeof(double) * 2 * h->nB_tot) * 2 + al256(sizeof(double) * h->nA_tot) +
+           al256(sizeof(CgState) * 2 * h->nprob) + al256(sizeof(double) * h->nprob);
+}
+
+int qf_pcg_solve_ragged(int prec, const void* plan_host, const void* plan_dev, const void* phi, const void* b, void* x, double dtau,
+                        double t, const void* ptab, double rtol, int max_iter, void* work, size_t work_bytes, double* err_out,
+                        double* iters_out, void* stream) {
+    const RaggedHead* h;
+    const Prob* pd;
+    int rc = ragged_head(plan_host, plan_dev, &h, &pd);
+    if (rc) return rc;
+    if (!phi || !b || !x || !work || !err_out || !iters_out) return fail(QF_ERR_INVALID, "null pointer");
+    if (max_iter < 1) return fail(QF_ERR_INVALID, "max_iter=%d must be >= 1", max_iter);
+    const size_t need = qf_pcg_ragged_workspace_bytes(prec, plan_host);
+    if (need == 0 || work_bytes < need) return fail(QF_ERR_INVALID, "workspace of %zu bytes, need %zu", work_bytes, need);
+    hipStream_t s = (hipStream_t)stream;
+    auto run = [&](auto tag) -> int {
+        using R = decltype(tag);
+        const size_t vb = al256((size_t)h->nsites * sizeof(cplx<R>));
+        char* p = static_cast<char*>(work);
+        size_t off = 0;
+        auto take = [&](size_t n) { void* r = p + off; off += al256(n); return r; };
+        void* Rb[2] = {take(vb), take(vb)};
+        void* Pb[2] = {take(vb), take(vb)};
+        void* Z = take(vb);
+        void* OP = take(vb);
+        void* HOP = take(2 * vb);
+        double* rzp = (double*)take(sizeof(double) * 2 * h->nB_tot);
+        double* rrp = (double*)take(sizeof(double) * 2 * h->nB_tot);
+        double* pap = (double*)take(sizeof(double) * h->nA_tot);
+        CgState* st = (CgState*)take(sizeof(CgState) * 2 * h->nprob);
+        double* normb = (double*)take(sizeof(double) * h->nprob);
+        const Hop<R> hp = make_hop<R>(dtau, t);
+        const int np = h->nprob;
+        const long long ps = h->nB_tot;
+        hipLaunchKernelGGL((hop_table_ragged_kernel<R>), dim3((unsigned)std::min<long long>((h->nsites + 255) / 256, 16384)), dim3(256),
+                           0, s, (const R*)phi, (cplx<R>*)HOP, pd, np, h->nsites, std::tanh(t * dtau), std::tanh(0.5 * t * dtau));
+        QF_LAUNCHED();
+        CguArgs<R> u{};
+        u.ptab = (const R*)ptab; u.prob = pd; u.nprob = np;
+        u.rold = (const cplx<R>*)b; u.x = (cplx<R>*)x; u.rnew = (cplx<R>*)Rb[1]; u.z = (cplx<R>*)Z;
+        u.st_init = st + np; u.rz_part = rzp + ps; u.rr_part = rrp + ps;
+        int rc2 = launch_cgu_ragged<R, CGU_INIT>(u, *h, s);
+        if (rc2) return rc2;
+        MdmArgs<R> m{};
+        m.phi = (const R*)phi; m.hop = (const cplx<R>*)HOP; m.hp = hp; m.prob = pd; m.nprob = np; m.nrhs = 1;
+        m.normb = normb; m.rtol = rtol; m.dot_part = pap; m.z = (const cplx<R>*)Z; m.y = (cplx<R>*)OP; m.may_stop = 1;
+        for (int i = 0; i < max_iter; ++i) {
+            const int cur = i & 1, prv = cur ^ 1;
+            m.iter = i;
+            m.rz_part = rzp + prv * ps; m.rr_part = rrp + prv * ps;
+            m.st_prev = st + prv * np; m.st_out = st + cur * np;
+            m.pold = (const cplx<R>*)Pb[prv]; m.pnew = (cplx<R>*)Pb[cur];
+            rc2 = launch_mdm_ragged<R, MDM_CG, true>(m, *h, s);
+            if (rc2) return rc2;
+            u.rold = (const cplx<R>*)Rb[prv]; u.op = (const cplx<R>*)OP; u.p = (const cplx<R>*)Pb[cur];
+            u.x = (cplx<R>*)x; u.rnew = (cplx<R>*)Rb[cur]; u.z = (cplx<R>*)Z;
+            u.pap = pap; u.st = st + cur * np;
+            u.rz_part = rzp + cur * ps; u.rr_part = rrp + cur * ps;
+            rc2 = launch_cgu_ragged<R, CGU_ITER>(u, *h, s);
+            if (rc2) return rc2;
+        }
+        const int last = (max_iter - 1) & 1;
+        hipLaunchKernelGGL(cg_finalize_kernel, dim3(np), dim3(64), 0, s, (const double*)(rrp + last * ps), 0,
+                           (const CgState*)(st + last * np), (const double*)normb, rtol, 0, err_out, iters_out, pd);
+        QF_LAUNCHED();
+        return QF_OK;
+    };
+    if (prec == QF_PREC_F32) return run(float{});
+    if (prec == QF_PREC_F64) return run(double{});
+    return fail(QF_ERR_INVALID, "prec %d", prec);
 }
 
 int qf_fermion_force(int prec, const void* phi, long long phi_cs, const void* xi, void* Ft, int nrhs, int L, int Ltau, double dtau,

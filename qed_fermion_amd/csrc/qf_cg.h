@@ -41,6 +41,8 @@ struct CguArgs {
     double* rz_part;       // [nrhs][nblk]
     double* rr_part;
     const int* ru_flag;    // REFRESH: skip chains whose reliable-update flag is 0 (nullptr: never skip)
+    const Prob* prob;      // ragged batch (nullptr: uniform; grid (nsx, tiles, nrhs))
+    int nprob;
 };
 
 // One thread per (site, slice) of a TX x TT tile; the new residual of the
@@ -54,17 +56,30 @@ __global__ void __launch_bounds__(TX * TT) cg_update_kernel(CguArgs<R> A) {
     constexpr int ROWS = TT + H;
     __shared__ cplx<R> ring[PRE ? ROWS * TX : 1];
     __shared__ double red[64];
-    const int rho = blockIdx.z;
-    const int Vs = A.Vs, Lt = A.Lt;
+    int rho, Vs, Lt, bx, by, blk, nA;
+    long long vo, pout, pin;
+    const R* ptab = A.ptab;
+    if (A.prob != nullptr) {   // ragged batch: one linear grid over every lattice's tiles
+        int k = 0;
+        for (int j = 1; j < A.nprob; ++j)
+            if (A.prob[j].bwg0 <= (int)blockIdx.x) k = j;
+        const Prob& P = A.prob[k];
+        rho = k; Vs = P.L * P.L; Lt = P.Lt; blk = blockIdx.x - P.bwg0;
+        bx = blk % P.nsx; by = blk / P.nsx;
+        vo = P.voff; pout = P.pb0 + blk; pin = P.pa0; nA = P.nA;
+        if (ptab) ptab += P.toff;
+    } else {
+        rho = blockIdx.z; Vs = A.Vs; Lt = A.Lt; bx = blockIdx.x; by = blockIdx.y;
+        blk = bx + gridDim.x * by;
+        vo = rho * A.vs; pout = (long long)rho * gridDim.x * gridDim.y + blk; pin = (long long)rho * A.nA; nA = A.nA;
+    }
     const int sx = threadIdx.x % TX, st = threadIdx.x / TX;
-    const int s = blockIdx.x * TX + sx;
-    const int t0 = blockIdx.y * TT;
+    const int s = bx * TX + sx;
+    const int t0 = by * TT;
     const int tau = t0 + st;
     const bool col = s < Vs;
     const bool own = col && tau < Lt;
     const int sc = col ? s : 0;   // loads of padding columns read site 0 (never stored)
-    const long long vo = rho * A.vs;
-    const int blk = blockIdx.x + gridDim.x * blockIdx.y;
 
     if constexpr (MODE == CGU_ITER) {
         if (A.st[rho].act == 0.0) return;   // converged chain: x and r stay frozen (uniform)
@@ -103,8 +118,8 @@ __global__ void __launch_bounds__(TX * TT) cg_update_kernel(CguArgs<R> A) {
     R alpha = R(0);
     if constexpr (MODE == CGU_ITER) {
         double sums[1];
-        const double* src[1] = {A.pap + (long long)rho * A.nA};
-        sum_partials<1>(src, A.nA, sums, red);
+        const double* src[1] = {A.pap + pin};
+        sum_partials<1>(src, nA, sums, red);
         alpha = (R)(A.st[rho].rz / sums[0]);
         ra = mk<R>(ra.x - alpha * oa.x, ra.y - alpha * oa.y);
         rb = mk<R>(rb.x - alpha * ob.x, rb.y - alpha * ob.y);
@@ -116,7 +131,7 @@ __global__ void __launch_bounds__(TX * TT) cg_update_kernel(CguArgs<R> A) {
         if (second) ring[(st + TT) * TX + sx] = rb;
         __syncthreads();
         r = ring[(st + H / 2) * TX + sx];
-        const R* c = A.ptab + min(tau, Lt - 1) * 13;
+        const R* c = ptab + min(tau, Lt - 1) * 13;
         zz = mk<R>(0, 0);
 #pragma unroll
         for (int k = 0; k < 13; ++k) {
@@ -141,9 +156,8 @@ __global__ void __launch_bounds__(TX * TT) cg_update_kernel(CguArgs<R> A) {
         double v[2] = {rz, rr};
         block_sum<2>(v, red);
         if (threadIdx.x == 0) {
-            const long long nblk = (long long)gridDim.x * gridDim.y;
-            A.rz_part[rho * nblk + blk] = v[0];
-            A.rr_part[rho * nblk + blk] = v[1];
+            A.rz_part[pout] = v[0];
+            A.rr_part[pout] = v[1];
         }
     }
 }
@@ -153,11 +167,16 @@ __global__ void __launch_bounds__(TX * TT) cg_update_kernel(CguArgs<R> A) {
 // One block per RHS.  If true_res, rr partials hold |b - A x|^2 in FP64 and the
 // error is reported from them even for converged chains.
 __global__ void cg_finalize_kernel(const double* rr_part, int n, const CgState* st_prev, const double* normb, double rtol,
-                                   int true_res, double* err_out, double* iters_out) {
+                                   int true_res, double* err_out, double* iters_out, const Prob* prob) {
     __shared__ double red[64];
     const int rho = blockIdx.x;
     double sums[1];
-    const double* src[1] = {rr_part + (long long)rho * n};
+    long long off = (long long)rho * n;
+    if (prob != nullptr) {   // ragged batch: this lattice's update partials
+        off = prob[rho].pb0;
+        n = prob[rho].nB;
+    }
+    const double* src[1] = {rr_part + off};
     sum_partials<1>(src, n, sums, red);
     if (threadIdx.x == 0) {
         const CgState S = st_prev[rho];

@@ -108,6 +108,31 @@ template <typename R> struct Hop {
     double dtau_, t_;   // the parameters they were built from
 };
 
+// One lattice of a ragged batch: independent lattices of different L solved
+// in the same launches, one right-hand side (chain) each.  Offsets are in
+// elements of the concatenated arrays.  The plan is a RaggedHead followed by
+// nprob Prob entries; the kernels read it from device memory and find their
+// lattice from the work-group index (wg0 / bwg0: first work-group of a
+// lattice in the operator / update launch).
+struct Prob {
+    int L, Lt;        // lattice
+    int T, nA;        // operator launch: tau slices per work-group, runs
+    int wg0;          // first operator work-group
+    int nsx, nB;      // update launch: site tiles per slice row, tiles in total
+    int bwg0;         // first update work-group
+    long long voff;   // vectors (complex)
+    long long poff;   // phi (reals)
+    long long hoff;   // hop table (complex)
+    long long toff;   // preconditioner table (13 * Lt reals)
+    int pa0, pb0;     // operator / update partial sums
+};
+struct RaggedHead {
+    int nprob, by, nthreads;    // patch height and work-group size of the operator launch
+    int nwg_op, nwg_upd, tb;    // launch sizes; update tile slices
+    int nA_tot, nB_tot;         // partial sums
+    long long nsites, ntab;     // total sites; preconditioner table length
+};
+
 // Per-RHS scalar state of the batched PCG (parity double-buffered).
 struct CgState {
     double rz;     // Re(r^dag z) of the last completed iteration

@@ -47,6 +47,8 @@ struct MdmArgs {
     long long vs;
     int L, Lt, T;
     int nrhs;               // set by the launcher (grid = runs * nrhs)
+    const Prob* prob;       // ragged batch: per-lattice shapes and offsets (nullptr: uniform)
+    int nprob;
     Hop<R> hp;
     double* dot_part;       // [nrhs][nblk]: ||(Mx)_tau||^2 = Re(x^dag M^dag M x) over the run (optional)
     // MODE 1 (CG step A): x_tau := z_tau + beta p_tau, written to pnew
@@ -114,11 +116,23 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
     // 1-D grid, right-hand side fastest: the work-groups dispatched together
     // (round-robin over the XCDs) march the same tau run, so a shared field's
     // coefficients are fetched into each XCD's L2 once for all of them.
-    const int rho = blockIdx.x % A.nrhs;
-    const int run = blockIdx.x / A.nrhs;
-    const int L = A.L, Lt = A.Lt, Vs = L * L;
-    const int t0 = run * A.T;
-    const int t1 = min(t0 + A.T, Lt);
+    int rho, run, L, Lt, T, nBk;
+    long long vo, pho, hoo, pa, pb;
+    if (A.prob != nullptr) {   // ragged batch: find this work-group's lattice
+        int k = 0;
+        for (int j = 1; j < A.nprob; ++j)
+            if (A.prob[j].wg0 <= (int)blockIdx.x) k = j;
+        const Prob& P = A.prob[k];
+        rho = k; run = blockIdx.x - P.wg0; L = P.L; Lt = P.Lt; T = P.T; nBk = P.nB;
+        vo = P.voff; pho = P.poff; hoo = P.hoff; pa = P.pa0; pb = P.pb0;
+    } else {
+        rho = blockIdx.x % A.nrhs; run = blockIdx.x / A.nrhs; L = A.L; Lt = A.Lt; T = A.T; nBk = A.nB;
+        vo = rho * A.vs; pho = rho * A.phi_cs; hoo = rho * A.hop_cs;
+        pa = (long long)rho * (gridDim.x / A.nrhs); pb = (long long)rho * A.nB;
+    }
+    const int Vs = L * L;
+    const int t0 = run * T;
+    const int t1 = min(t0 + T, Lt);
     const Geo g = make_geo<BY>(L, threadIdx.x);
     const PatchOff po = patch_off<R>(g);
     const XAddr xa = make_xaddr<BY>(g);
@@ -127,9 +141,8 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
     if constexpr (MODE == MDM_RESID) {
         if (A.ru_flag != nullptr && A.ru_flag[rho] == 0) return;   // uniform over the block
     }
-    const long long vo = rho * A.vs;
-    const R* phi = A.phi + rho * A.phi_cs;
-    const __amdgpu_buffer_rsrc_t rs_h = HOPT ? buf_rsrc(A.hop + rho * A.hop_cs, 2u * vbytes) : buf_rsrc(A.x, 0u);
+    const R* phi = A.phi + pho;
+    const __amdgpu_buffer_rsrc_t rs_h = HOPT ? buf_rsrc(A.hop + hoo, 2u * vbytes) : buf_rsrc(A.x, 0u);
     // x-like inputs: MODE_CG reads z and p_old (x = z + beta p_old), else x
     const bool cg = MODE == MDM_CG;
     const __amdgpu_buffer_rsrc_t rs_a = buf_rsrc(cg ? (const void*)(A.z + vo) : (const void*)(A.x + vo), vbytes);
@@ -172,8 +185,8 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
         // every block (hmc_sampler_batch.py:806-844: error, active mask,
         // beta = rz_new / rz_old).
         double sums[2];
-        const double* src[2] = {A.rz_part + (long long)rho * A.nB, A.rr_part + (long long)rho * A.nB};
-        sum_partials<2>(src, A.nB, sums, red);
+        const double* src[2] = {A.rz_part + pb, A.rr_part + pb};
+        sum_partials<2>(src, nBk, sums, red);
         const CgState St = A.st_prev[rho];
         const double nb = (A.iter == 0) ? sqrt(sums[1]) : A.normb[rho];
         double act, err, rzv, it = St.iters, bt = 0.0;
@@ -365,12 +378,12 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
         // fold the touched dwords in; T < 0 never holds, so this only keeps the loads alive
 #pragma unroll
         for (int k = 0; k < NTOUCH; ++k) tacc ^= td[k];
-        if (A.T < 0 && tacc == 0x7fc00001u) dsum = 0.0;
+        if (T < 0 && tacc == 0x7fc00001u) dsum = 0.0;
     }
     if (A.dot_part != nullptr) {
         double v[1] = {dsum};
         block_sum<1>(v, red);
-        if (threadIdx.x == 0) A.dot_part[(long long)rho * (gridDim.x / A.nrhs) + run] = v[0];
+        if (threadIdx.x == 0) A.dot_part[pa + run] = v[0];
     }
 }
 

@@ -634,4 +634,34 @@ __global__ void hop_table_kernel(const R* __restrict__ phi, long long phi_cs, cp
     }
 }
 
+// Hop tables of a ragged batch (every lattice's table in one launch; site i of
+// the concatenated vectors belongs to the last lattice with voff <= i).
+template <typename R>
+__global__ void hop_table_ragged_kernel(const R* __restrict__ phi, cplx<R>* __restrict__ hop, const Prob* __restrict__ prob, int nprob,
+                                        long long nsites, double th1, double th2) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nsites; i += (long long)gridDim.x * blockDim.x) {
+        int k = 0;
+        for (int j = 1; j < nprob; ++j)
+            if (prob[j].voff <= i) k = j;
+        const Prob& P = prob[k];
+        const long long loc = i - P.voff;
+        const int Vs = P.L * P.L;
+        const int s = (int)(loc % Vs);
+        const int x = s % P.L, y = s / P.L;
+        const R* ph = phi + P.poff + 2 * loc;
+        cplx<R>* h = hop + P.hoff + 2 * loc;
+        R sx, cx, sy, cy;
+        if constexpr (sizeof(R) == 4) {
+            sincosf(ph[0], &sx, &cx);
+            sincosf(ph[1], &sy, &cy);
+        } else {
+            sincos(ph[0], &sx, &cx);
+            sincos(ph[1], &sy, &cy);
+        }
+        const R tx = ((x + y) & 1) ? (R)th2 : (R)th1;
+        h[0] = mk<R>(tx * cx, tx * sx);
+        h[1] = mk<R>((R)th2 * cy, (R)th2 * sy);
+    }
+}
+
 }  // namespace qf

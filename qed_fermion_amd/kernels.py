@@ -267,3 +267,68 @@ def get_tiling(prec, nrhs, L, Ltau):
     out = (ctypes.c_int * 4)()
     call("qf_get_tiling", int(prec), nrhs, L, Ltau, out)
     return {"by": out[0], "t": out[1], "tb": out[2], "nA": out[3]}
+
+
+# ------------------------------------------------------------------ ragged batch
+class RaggedPlan:
+    """Plan of a ragged batch: independent lattices of different L (one chain
+    each) solved in the same launches.  Holds the host plan and its device copy
+    (the C ABI takes both)."""
+
+    def __init__(self, Ls, Ltaus, device="cuda"):
+        import numpy as np
+        self.Ls = [int(v) for v in Ls]
+        self.Ltaus = [int(v) for v in Ltaus]
+        if len(self.Ls) != len(self.Ltaus) or not self.Ls:
+            raise QfError(_lib.QF_ERR_INVALID, "L and Ltau lists must be non-empty and of equal length")
+        n = len(self.Ls)
+        nbytes = _lib.lib().qf_ragged_plan_bytes(n)
+        self.host = np.zeros(nbytes, dtype=np.uint8)
+        Li = np.asarray(self.Ls, dtype=np.int32)
+        Lti = np.asarray(self.Ltaus, dtype=np.int32)
+        call("qf_ragged_plan", n, Li.ctypes.data, Lti.ctypes.data, self.host.ctypes.data, nbytes)
+        self.dev = torch.from_numpy(self.host.copy()).to(device)
+        self.sizes = [L * L * Lt for L, Lt in zip(self.Ls, self.Ltaus)]
+        self.offsets = [0]
+        for v in self.sizes:
+            self.offsets.append(self.offsets[-1] + v)
+        self.nsites = self.offsets[-1]
+
+    def split(self, v):
+        """Per-lattice views of a concatenated vector (complex: one value per site)."""
+        return [v[self.offsets[k]:self.offsets[k + 1]] for k in range(len(self.Ls))]
+
+    def precon_table(self, dtype=torch.float64, device="cuda"):
+        """The manual preconditioner's band tables of every lattice, concatenated."""
+        return torch.cat([precon_table(Lt, dtype=dtype, device=device).reshape(-1) for Lt in self.Ltaus])
+
+
+def apply_ragged(plan, phi, x, dtau=0.1, t=1.0, out=None):
+    """y = M^dag M x for every lattice of a ragged batch in one launch.
+    phi: concatenated [2 * sum N_k] real; x: [sum N_k] complex."""
+    _check_dev(phi, x)
+    if x.numel() != plan.nsites or phi.numel() != 2 * plan.nsites:
+        raise QfError(_lib.QF_ERR_INVALID, "ragged vector / field sizes do not match the plan")
+    y = torch.empty_like(x) if out is None else out
+    call("qf_apply_ragged", _lib.QF_OP_MDAGM, _prec(x), plan.host.ctypes.data, ptr(plan.dev), ptr(phi), ptr(x), ptr(y),
+         float(dtau), float(t), stream_handle())
+    return y
+
+
+def pcg_solve_ragged(plan, phi, b, dtau=0.1, t=1.0, ptab=None, rtol=1e-9, max_iter=400, x=None):
+    """Batched PCG over a ragged batch.  Returns (x, iters, err) with one
+    iters / err entry per lattice."""
+    _check_dev(phi, b, ptab)
+    if b.numel() != plan.nsites or phi.numel() != 2 * plan.nsites:
+        raise QfError(_lib.QF_ERR_INVALID, "ragged vector / field sizes do not match the plan")
+    prec = _prec(b)
+    nbytes = _lib.lib().qf_pcg_ragged_workspace_bytes(prec, plan.host.ctypes.data)
+    work = _workspace(("ragged", prec, tuple(plan.Ls), tuple(plan.Ltaus), b.device), nbytes, b.device)
+    x = torch.empty_like(b) if x is None else x
+    n = len(plan.Ls)
+    err = torch.empty(n, dtype=torch.float64, device=b.device)
+    iters = torch.empty(n, dtype=torch.float64, device=b.device)
+    tab = ptab.to(_REAL[b.dtype]).contiguous() if ptab is not None else None
+    call("qf_pcg_solve_ragged", prec, plan.host.ctypes.data, ptr(plan.dev), ptr(phi), ptr(b), ptr(x), float(dtau), float(t),
+         ptr(tab), float(rtol), int(max_iter), ptr(work), work.numel(), ptr(err), ptr(iters), stream_handle())
+    return x, iters, err

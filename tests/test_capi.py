@@ -74,3 +74,26 @@ def test_argument_validation_without_gpu():
     out = (ctypes.c_int * 4)()
     assert h.qf_get_tiling(0, 1, 36, 360, out) == 0 and out[0] in (2, 4, 6)
     assert h.qf_get_tiling(0, 1, 66, 660, out) == 0 and out[0] == 6
+
+
+def test_ragged_plan_layout():
+    """qf_ragged_plan (host only): offsets, sizes and argument checks."""
+    import ctypes
+    import numpy as np
+    from qed_fermion_amd import _lib
+    lib = _lib.lib()
+    Ls = np.array([12, 18, 24, 30], dtype=np.int32)
+    Lts = np.array([120, 180, 240, 300], dtype=np.int32)
+    n = len(Ls)
+    nb = lib.qf_ragged_plan_bytes(n)
+    assert nb > 0
+    plan = np.zeros(nb, dtype=np.uint8)
+    assert lib.qf_ragged_plan(n, Ls.ctypes.data, Lts.ctypes.data, plan.ctypes.data, nb) == 0
+    head = np.frombuffer(plan[:48].tobytes(), dtype=np.int32)
+    assert head[0] == n and head[1] == 2          # nprob, patch height
+    assert head[2] == 256                          # 15 x 15 = 225 threads -> 4 waves
+    nsites = np.frombuffer(plan[32:48].tobytes(), dtype=np.int64)
+    assert nsites[0] == int(np.sum(Ls.astype(np.int64) ** 2 * Lts)) and nsites[1] == 13 * int(np.sum(Lts))
+    odd = np.array([12, 13], dtype=np.int32)
+    assert lib.qf_ragged_plan(2, odd.ctypes.data, Lts.ctypes.data, plan.ctypes.data, nb) == _lib.QF_ERR_ODD_L
+    assert lib.qf_ragged_plan(n, Ls.ctypes.data, Lts.ctypes.data, plan.ctypes.data, 8) == _lib.QF_ERR_INVALID

@@ -1,0 +1,64 @@
+"""Ragged batch: eight lattices of L in {12, 18, 24, 30} (Ltau = 10 L, twice
+each) in the same launches -- the batching configuration's ragged set."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import make_phi, make_vec, relerr
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+LS = [12, 18, 24, 30, 12, 18, 24, 30]
+
+
+def _batch(kind, seed):
+    phis, vecs = [], []
+    for k, L in enumerate(LS):
+        phis.append(make_phi(L, 10 * L, kind, seed=seed + k).reshape(-1))
+        vecs.append(make_vec(L, 10 * L, seed=seed + 100 + k).reshape(-1))
+    return phis, vecs
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 2e-5)])
+def test_apply_ragged_matches_oracle(dev, prec, tol):
+    from qed_fermion_amd import kernels as K
+    print(f"[ragged apply] {prec}", flush=True)
+    plan = K.RaggedPlan(LS, [10 * L for L in LS])
+    phis, vecs = _batch("uniform", 5)
+    rd, cd = (torch.float64, torch.complex128) if prec == "f64" else (torch.float32, torch.complex64)
+    phi = torch.from_numpy(np.concatenate(phis)).to(dev, rd)
+    x = torch.from_numpy(np.concatenate(vecs)).to(dev, cd)
+    for _ in range(3):   # repeated launches of the same batch
+        y = K.apply_ragged(plan, phi, x).cpu().numpy()
+        for k, (L, yk) in enumerate(zip(LS, plan.split(y))):
+            ref = O.apply_MdagM(phis[k].reshape(10 * L, L, L, 2), vecs[k], L, 10 * L)
+            assert relerr(yk, ref) < tol, (k, L)
+
+
+def test_pcg_ragged_matches_per_lattice_solves(dev):
+    from qed_fermion_amd import kernels as K
+    print("[ragged pcg]", flush=True)
+    plan = K.RaggedPlan(LS, [10 * L for L in LS])
+    phis, vecs = _batch("gauss", 11)
+    phi = torch.from_numpy(np.concatenate(phis)).to(dev, torch.float64)
+    R = torch.from_numpy(np.concatenate(vecs)).to(dev, torch.complex128)
+    # b_k = M_k^dag R_k per lattice
+    b = torch.cat([K.apply("Mdag", p.view(1, -1), r.view(1, -1), L, 10 * L).view(-1)
+                   for p, r, L in zip(plan.split(phi.view(-1, 2)), plan.split(R), LS)]).contiguous()
+    ptab = plan.precon_table(device=dev)
+    x, iters, err = K.pcg_solve_ragged(plan, phi, b, ptab=ptab, rtol=1e-10, max_iter=2000)
+    xs = plan.split(x)
+    for k, L in enumerate(LS):
+        Lt = 10 * L
+        pk = plan.split(phi.view(-1, 2))[k].reshape(1, -1)
+        bk = plan.split(b)[k].view(1, -1)
+        # the same solve alone (uniform path): same recurrence; the sums are
+        # reduced in another order, so the stopping iteration may differ by one
+        xk, itk, ek = K.pcg_solve(pk, bk, L, Lt, ptab=K.precon_table(Lt), rtol=1e-10, max_iter=2000)
+        assert abs(iters[k].item() - itk.item()) <= 1, (k, iters[k].item(), itk.item())
+        assert relerr(xs[k].cpu().numpy(), xk.view(-1).cpu().numpy()) < 1e-8, k
+        # the oracle's true residual of the ragged solution
+        assert err[k].item() <= 1e-10 and iters[k].item() < 2000, (k, err[k].item())
+        r = O.apply_MdagM(pk.cpu().numpy().reshape(Lt, L, L, 2), xs[k].cpu().numpy(), L, Lt)
+        assert relerr(r, bk.view(-1).cpu().numpy()) <= 1.01e-10, k
