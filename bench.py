@@ -6,7 +6,8 @@ the reference's manual tau-band preconditioner, PCG with the reference's
 graph-mode semantics (fixed max_iter per solve, cg_rtol; converged chains
 freeze).  A "step" is one HMC trajectory of a batch of --bs chains, the whole
 trajectory replayed from one hipGraph (mixed precision: complex64 PCG with
-FP64 dots and FP64 reliable updates every 50 iterations).
+FP64 dots and adaptive FP64 reliable updates, at most 50 iterations apart).
+Tilings are picked by the autotuner (qed_fermion_amd/autotune.py) first.
 
 Reported next to it:
   * the same-GPU FP64 plain path: complex128 PCG, one chain per trajectory,
@@ -16,6 +17,11 @@ Reported next to it:
   * M^dag M (mdagm_kernel, complex64) GB/s from HIP events around back-to-back
     launches on the launch stream, for --roof-nrhs right-hand sides sharing phi
     (working set far above the 256 MB Infinity Cache), vs the 8 TB/s HBM peak;
+  * M^dag M as the trajectory applies it (each chain its own field);
+  * other_configs: the batching configuration (stochastic-estimator solve at
+    L=48, Ltau=480 with 128 Z2 noise vectors on one field; the ragged set of 8
+    lattices L in {12,18,24,30} x 2 in one launch per kernel) and a 20-step
+    mixed-precision trajectory at L=66, Ltau=660;
   * cpu_baseline: the FP64 OpenMP oracle (oracle/qf_oracle.c, a C port of the
     reference's PCG/force path) timed on this host for a bounded number of PCG
     iterations and scaled to one trajectory.
@@ -46,17 +52,23 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--L", type=int, default=36)
     ap.add_argument("--Ltau", type=int, default=360)
-    ap.add_argument("--bs", type=int, default=4)
+    ap.add_argument("--bs", type=int, default=8)
     ap.add_argument("--max-iter", type=int, default=400)
     ap.add_argument("--rtol", type=float, default=1e-9)
     ap.add_argument("--ru-period", type=int, default=50)
     ap.add_argument("--n-leap", type=int, default=5)
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--roof-nrhs", type=int, default=64)
+    ap.add_argument("--roof-chains", type=int, default=32,
+                    help="chains (own fields) of the roofline M^dag M; 32 at L=36 is a 477 MB working set")
     ap.add_argument("--roof-reps", type=int, default=20)
     ap.add_argument("--cpu-iters", type=int, default=0, help="oracle PCG iterations for cpu_baseline (0 = auto, ~15 s)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="use the built-in tilings instead of the autotuner")
+    ap.add_argument("--probe", choices=["per_chain", "shared"], default=None,
+                    help="only launch a calibration copy and the roofline M^dag M three times each (for rocprofv3 --pmc)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batching (L=48, 128 Z2 RHS; ragged set) and L=66 trajectory measurements")
     ap.add_argument("--seed", type=int, default=2024)
     return ap.parse_args()
 
@@ -96,11 +108,75 @@ def mdagm_roofline(torch, K, L, Lt, nrhs, reps, dev, use_hop=True):
             "nrhs": nrhs, "ms_per_launch": ms, "bytes_per_launch": algo, "GBps": algo / (ms * 1e-3) / 1e9}
 
 
-def traffic_of(L, Lt, nrhs):
-    """HBM bytes per launch of the roofline kernel at this shape, as measured by
-    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by a copy-kernel
-    calibration) and recorded in profiles/mdagm_traffic.json; None if that
-    shape was not measured."""
+def timed(torch, fn, reps=1):
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def extras(torch, K, args, dev):
+    """The other configurations, one measurement each (not the headline)."""
+    import numpy as np
+    from qed_fermion_amd import autotune
+    from qed_fermion_amd.trajectory import TrajectoryEngine
+    out = {}
+    # -- batching: stochastic-estimator solve, L=48, Ltau=480, 128 Z2 noise vectors, one field
+    L, Lt, nrv = 48, 480, 128
+    N = L * L * Lt
+    g = torch.Generator(device="cpu").manual_seed(48)
+    phi = ((torch.rand(1, 2 * N, generator=g, dtype=torch.float64) * 2 - 1) * math.pi).to(dev, torch.float32)
+    eta = (torch.randint(0, 2, (nrv, N), generator=g) * 2 - 1).to(torch.float32)
+    eta = (torch.complex(eta, (torch.randint(0, 2, (nrv, N), generator=g) * 2 - 1).to(torch.float32)) / math.sqrt(2)).to(dev)
+    if not args.no_tune:
+        autotune.tune(0, L, Lt, nrv, kind="operator", persist=False)
+    b = K.apply("Mdag", phi, eta, L, Lt)
+    ptab = K.precon_table(Lt, dtype=torch.float32, device=dev)
+    x = torch.empty_like(b)
+    it = args.max_iter
+    t_se = timed(torch, lambda: K.pcg_solve(phi, b, L, Lt, ptab=ptab, rtol=0.0, max_iter=it, x=x))
+    hop = K.hop_table(phi, L, Lt)
+    t_op = timed(torch, lambda: K.apply_hop("MdagM", hop, eta, L, Lt, out=x), reps=5)
+    out["se_L48_128rhs"] = {"solve_s": t_se, "iterations": it, "ms_per_iteration": t_se / it * 1e3,
+                            "mdagm_GBps": (nrv * 16 * N + 16 * N) / t_op / 1e9, "mdagm_ms": t_op * 1e3}
+    log(f"[bench] SE L=48 128 RHS: {t_se:.3f} s per {it}-iteration solve, M^dag M {out['se_L48_128rhs']['mdagm_GBps']:.0f} GB/s")
+    del eta, b, x, hop
+    # -- batching: ragged set, L in {12, 18, 24, 30} x 2, Ltau = 10 L, one launch per kernel
+    Ls = [12, 18, 24, 30, 12, 18, 24, 30]
+    plan = K.RaggedPlan(Ls, [10 * v for v in Ls], device=dev)
+    phr = ((torch.rand(2 * plan.nsites, generator=g, dtype=torch.float64) * 2 - 1) * math.pi).to(dev, torch.float32)
+    br = torch.randn(plan.nsites, generator=g, dtype=torch.complex128).to(dev, torch.complex64)
+    ptr_ = plan.precon_table(dtype=torch.float32, device=dev)
+    t_rg = timed(torch, lambda: K.pcg_solve_ragged(plan, phr, br, ptab=ptr_, rtol=0.0, max_iter=it))
+    t_rop = timed(torch, lambda: K.apply_ragged(plan, phr, br), reps=10)
+    out["ragged_8_lattices"] = {"L": Ls, "solve_s": t_rg, "iterations": it, "ms_per_iteration": t_rg / it * 1e3,
+                                "mdagm_ms": t_rop * 1e3, "mdagm_GBps": 24 * plan.nsites / t_rop / 1e9}
+    log(f"[bench] ragged batch: {t_rg:.3f} s per {it}-iteration solve, one M^dag M launch {t_rop * 1e3:.3f} ms")
+    # -- differentiation at full size: L=66, Ltau=660, a 20-step mixed trajectory
+    L, Lt = 66, 660
+    N = L * L * Lt
+    rng = np.random.default_rng(66)
+    ph66 = torch.from_numpy(rng.uniform(-np.pi, np.pi, (1, 2 * N))).to(dev)
+    eng = TrajectoryEngine(L, Lt, 1, precision="mixed", ru_period=args.ru_period, dtau=0.1, J=1.0, K=1.0, compact=False,
+                           m=1.0, delta_t=0.06, n_leap=20, max_iter=args.max_iter, rtol=args.rtol, device=dev)
+    eng.set_phi(ph66)
+    eng.capture(n_warmup=1)
+    t66 = timed(torch, eng.replay)
+    out["trajectory_L66_20step_mixed"] = {"s_per_trajectory": t66, "solves": 21, "max_iter": args.max_iter,
+                                          "max_residual": eng.err.max().item()}
+    log(f"[bench] L=66 20-step mixed trajectory: {t66:.3f} s")
+    del eng
+    return out
+
+
+def traffic_of(L, Lt, nrhs, kind):
+    """HBM bytes per launch of an M^dag M measurement at this shape, as
+    measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (each scaled by the
+    same counter's reading for a copy of known size) and recorded in
+    profiles/mdagm_traffic.json; None if that shape was not measured."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "mdagm_traffic.json")
     try:
         with open(path) as f:
@@ -108,16 +184,86 @@ def traffic_of(L, Lt, nrhs):
     except (OSError, ValueError):
         return None
     for e in rec.get("entries", []):
-        if (e.get("L"), e.get("Ltau"), e.get("nrhs")) == (L, Lt, nrhs):
+        if (e.get("L"), e.get("Ltau"), e.get("nrhs"), e.get("kind")) == (L, Lt, nrhs, kind):
             return e.get("traffic_bytes")
     return None
+
+
+def mdagm_per_chain(torch, K, L, Lt, nchain, reps, dev, tune=True):
+    """M^dag M as the trajectory applies it: nchain chains, each with its own
+    field (hop table).  Algorithmic bytes per launch: nchain * (8N read x +
+    8N write y + 16N read hop table).  Timed with HIP events on the launch
+    stream (torch's current stream) around back-to-back launches."""
+    N = L * L * Lt
+    if tune:
+        from qed_fermion_amd import autotune
+        autotune.tune(0, L, Lt, nchain, kind="operator", shared_field=False, persist=False)
+    g = torch.Generator(device="cpu").manual_seed(8)
+    phi = ((torch.rand(nchain, 2 * N, generator=g, dtype=torch.float64) * 2 - 1) * math.pi).to(dev, torch.float32)
+    x = torch.randn(nchain, N, dtype=torch.complex64, device=dev)
+    y = torch.empty_like(x)
+    hop = K.hop_table(phi, L, Lt)
+    run = lambda: K.apply_hop("MdagM", hop, x, L, Lt, out=y)
+    for _ in range(3):
+        run()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        run()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    algo = nchain * 32 * N
+    return {"kernel": "mdagm_kernel<float> (M^dag M, complex64, each chain its own field, from its hop table)",
+            "nchains": nchain, "ms_per_launch": ms, "bytes_per_launch": algo, "GBps": algo / (ms * 1e-3) / 1e9,
+            "tiling": K.get_tiling(0, nchain, L, Lt)}
+
+
+def probe(args):
+    """Three launches of a plain copy of known size, then three of the roofline
+    M^dag M (per-chain fields: --roof-chains chains; shared: --roof-nrhs RHS on
+    one field), tuned as in the full run.  Under rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE the copy's counters give the scale of the kernel's."""
+    import torch
+    from qed_fermion_amd import autotune
+    from qed_fermion_amd import kernels as K
+    L, Lt, dev = args.L, args.Ltau, torch.device("cuda:0")
+    N = L * L * Lt
+    shared = args.probe == "shared"
+    n = args.roof_nrhs if shared else args.roof_chains
+    if not args.no_tune:
+        autotune.tune(0, L, Lt, n, kind="operator", shared_field=shared, persist=False)
+    g = torch.Generator(device="cpu").manual_seed(7)
+    phi = ((torch.rand(1 if shared else n, 2 * N, generator=g, dtype=torch.float64) * 2 - 1) * math.pi).to(dev, torch.float32)
+    x = torch.randn(n, N, dtype=torch.complex64, device=dev)
+    y = torch.empty_like(x)
+    hop = K.hop_table(phi, L, Lt)
+    xc = torch.empty(n * N * 2, dtype=torch.float32, device=dev).uniform_()
+    yc = torch.empty_like(xc)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        yc.copy_(xc)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        K.apply_hop("MdagM", hop, x, L, Lt, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": args.probe, "L": L, "Ltau": Lt, "n": n, "copy_bytes_each_way": xc.numel() * 4,
+                      "algo_read": n * 8 * N + (1 if shared else n) * 16 * N, "algo_write": n * 8 * N,
+                      "tiling": K.get_tiling(0, n, L, Lt)}), flush=True)
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (see DESIGN.md, sharding)"}))
+        print(json.dumps({"error": "this build runs on one GPU only (--gpus must be 1): chains, pseudofermions and "
+                                   "noise vectors would shard across GPUs without communication, but no multi-GPU "
+                                   "path is built"}))
         sys.exit(2)
+    if args.probe:
+        probe(args)
+        return
     import numpy as np
     import torch
 
@@ -207,6 +353,10 @@ def main():
     log(f"[bench] M^dag M nrhs={args.roof_nrhs}: {roof['GBps']:.0f} GB/s (hop table), {roof_phi['GBps']:.0f} GB/s (phi); "
         f"nrhs={bs}: {roof_b['GBps']:.0f} GB/s")
 
+    per_chain = mdagm_per_chain(torch, K, L, Lt, args.roof_chains, args.roof_reps, dev, tune=not args.no_tune)
+    log(f"[bench] M^dag M, {args.roof_chains} chains with their own fields: {per_chain['GBps']:.0f} GB/s")
+    extra = None if args.no_extras else extras(torch, K, args, dev)
+
     # ---------------------------------------------------------- CPU baseline (oracle)
     cpu = None
     if not args.skip_cpu:
@@ -255,10 +405,16 @@ def main():
         "accuracy_vs_fp64": acc_report,
         "mdagm_batch_GBps": roof_b["GBps"],
         "mdagm_from_phi_GBps": roof_phi["GBps"],
-        "roofline": {"bound": "hbm", "achieved": roof["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": roof["GBps"] / HBM_PEAK_GBS, "traffic": traffic_of(L, Lt, roof["nrhs"]),
-                     "kernel": roof["kernel"], "nrhs": roof["nrhs"], "bytes_per_launch": roof["bytes_per_launch"],
-                     "ms_per_launch": roof["ms_per_launch"]},
+        "mdagm_per_chain": per_chain,
+        "other_configs": extra,
+        "roofline": {"bound": "hbm", "achieved": per_chain["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": per_chain["GBps"] / HBM_PEAK_GBS, "traffic": traffic_of(L, Lt, per_chain["nchains"], "per_chain"),
+                     "kernel": per_chain["kernel"], "nchains": per_chain["nchains"],
+                     "bytes_per_launch": per_chain["bytes_per_launch"], "ms_per_launch": per_chain["ms_per_launch"],
+                     "tiling": per_chain["tiling"]},
+        "mdagm_shared_field": {"GBps": roof["GBps"], "frac": roof["GBps"] / HBM_PEAK_GBS, "kernel": roof["kernel"],
+                               "nrhs": roof["nrhs"], "bytes_per_launch": roof["bytes_per_launch"],
+                               "ms_per_launch": roof["ms_per_launch"], "traffic": traffic_of(L, Lt, roof["nrhs"], "shared")},
         "tilings": tilings,
         "cpu_baseline": cpu,
     }
