@@ -53,7 +53,8 @@ const char* qf_last_error(void);
 
 /* Work-group tiling: patch height by in {2,4,6} (0 = auto), tau slices per
  * operator work-group t (0 = auto), tau slices of a PCG update tile tb in
- * {16, 32, 64} (0 = auto; the tile is tb slices x 1024/tb sites).  Set by the autotuner (qed_fermion_amd/autotune.py); no
+ * {16, 32, 64, 128} (0 = auto; 512-thread tiles of 32 sites for tb = 16, 16
+ * sites otherwise).  Set by the autotuner (qed_fermion_amd/autotune.py); no
  * counterpart in the reference, whose kernels take BLOCK_SIZE=(4,8)
  * (hmc_sampler_batch.py:111). */
 int qf_set_tuning(int prec, int by, int t, int tb);

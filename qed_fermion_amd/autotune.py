@@ -7,7 +7,7 @@ Three shape parameters decide how a launch maps onto the MI355X:
 * ``t``   -- tau slices marched by one operator work-group (the tau pipeline
   pays one extra B application per run, so long runs are cheaper, but fewer
   work-groups fill fewer CUs);
-* ``tb``  -- tau slices of a PCG update tile (16, 32 or 64; the tile's
+* ``tb``  -- tau slices of a PCG update tile (16, 32, 64 or 128; the tile's
   13-slice preconditioner window re-reads 12 halo slices).
 
 The reference picks a fixed CUDA block (BLOCK_SIZE = (4, 8),
@@ -103,7 +103,7 @@ def tune_operator(prec, L, Lt, nrhs, shared_field=True, reps=10, seed=0):
     return by, t, ms, table
 
 
-def tune_pcg(prec, L, Lt, nrhs, iters=20, reps=3, seed=0, tb_list=(16, 32, 64)):
+def tune_pcg(prec, L, Lt, nrhs, iters=20, reps=3, seed=0, tb_list=(16, 32, 64, 128)):
     """Best (by, t, tb) for the batched PCG iteration (per-chain fields):
     times ``iters`` iterations of pcg_solve with rtol = 0 (no chain stops).
     Returns (by, t, tb, ms_per_iteration, table)."""

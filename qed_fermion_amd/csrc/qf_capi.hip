@@ -76,9 +76,11 @@ struct Tiling {
     int TB, nsx, nB;   // update kernel: tile of TB slices x cgu_tx(TB) sites
 };
 
-// Update-kernel tiles are 1024 threads: TB in {16, 32, 64} tau slices.
-inline bool tb_ok(int tb) { return tb == 16 || tb == 32 || tb == 64; }
-inline int cgu_tx(int tb) { return 1024 / tb; }
+// Update-kernel tiles: 512 threads over TB in {16, 32, 64, 128} tau slices of
+// cgu_tx(TB) sites (32 sites for TB = 16, else 16 sites: 128-byte rows in
+// complex64); TB >= 64 gives each thread TB / 32 slices.
+inline bool tb_ok(int tb) { return tb == 16 || tb == 32 || tb == 64 || tb == 128; }
+inline int cgu_tx(int tb) { return tb == 16 ? 32 : 16; }
 
 // Tuning that applies to (prec, L, Lt, nrhs): exact shape, then any batch
 // size, then the per-precision setting.
@@ -124,9 +126,9 @@ Tiling tiling(int prec, int nrhs, int L, int Lt) {
     t.T = std::min(t.T, Lt);
     t.nA = (Lt + t.T - 1) / t.T;
     const int Vs = L * L;
-    // update kernel tile: TB tau slices x (1024 / TB) sites; the 12-slice
+    // update kernel tile: TB tau slices x cgu_tx(TB) sites; the 12-slice
     // preconditioner halo costs 12 / TB extra reads of r and M^dag M p
-    t.TB = tu.tb ? tu.tb : (Lt <= 16 ? 16 : 32);
+    t.TB = tu.tb ? tu.tb : (Lt >= 128 ? 128 : Lt >= 64 ? 64 : Lt > 16 ? 32 : 16);
     t.nsx = (Vs + cgu_tx(t.TB) - 1) / cgu_tx(t.TB);
     t.nB = t.nsx * ((Lt + t.TB - 1) / t.TB);
     (void)units;
@@ -285,11 +287,12 @@ int launch_force_by(const ForceArgs<R>& a, int nrhs, hipStream_t s) {
 
 template <typename R, int MODE, bool PRE>
 int launch_cgu_p(const CguArgs<R>& a, const Tiling& t, int nrhs, hipStream_t s) {
-    const dim3 grid(t.nsx, (a.Lt + t.TB - 1) / t.TB, nrhs), blk(1024);
+    const dim3 grid(t.nsx, (a.Lt + t.TB - 1) / t.TB, nrhs), blk(512);
     switch (t.TB) {
-        case 16: hipLaunchKernelGGL((cg_update_kernel<R, MODE, PRE, 64, 16>), grid, blk, 0, s, a); break;
-        case 32: hipLaunchKernelGGL((cg_update_kernel<R, MODE, PRE, 32, 32>), grid, blk, 0, s, a); break;
-        case 64: hipLaunchKernelGGL((cg_update_kernel<R, MODE, PRE, 16, 64>), grid, blk, 0, s, a); break;
+        case 16: hipLaunchKernelGGL((cg_update_kernel<R, MODE, PRE, 32, 16, 1>), grid, blk, 0, s, a); break;
+        case 32: hipLaunchKernelGGL((cg_update_kernel<R, MODE, PRE, 16, 32, 1>), grid, blk, 0, s, a); break;
+        case 64: hipLaunchKernelGGL((cg_update_kernel<R, MODE, PRE, 16, 64, 2>), grid, blk, 0, s, a); break;
+        case 128: hipLaunchKernelGGL((cg_update_kernel<R, MODE, PRE, 16, 128, 4>), grid, blk, 0, s, a); break;
         default: return fail(QF_ERR_UNSUPPORTED, "update tile of %d slices", t.TB);
     }
     QF_LAUNCHED();
@@ -326,11 +329,11 @@ int launch_mdm_ragged(const MdmArgs<R>& a, const RaggedHead& h, hipStream_t s) {
 
 template <typename R, int MODE>
 int launch_cgu_ragged(const CguArgs<R>& a, const RaggedHead& h, hipStream_t s) {
-    const dim3 grid((unsigned)h.nwg_upd), blk(1024);
+    const dim3 grid((unsigned)h.nwg_upd), blk(512);
     if (a.ptab)
-        hipLaunchKernelGGL((cg_update_kernel<R, MODE, true, 32, 32>), grid, blk, 0, s, a);
+        hipLaunchKernelGGL((cg_update_kernel<R, MODE, true, 16, 64, 2>), grid, blk, 0, s, a);
     else
-        hipLaunchKernelGGL((cg_update_kernel<R, MODE, false, 32, 32>), grid, blk, 0, s, a);
+        hipLaunchKernelGGL((cg_update_kernel<R, MODE, false, 16, 64, 2>), grid, blk, 0, s, a);
     QF_LAUNCHED();
     return QF_OK;
 }
@@ -544,7 +547,7 @@ const char* qf_last_error(void) { return g_err.c_str(); }
 int qf_set_tuning(int prec, int by, int t, int tb) {
     if (prec != QF_PREC_F32 && prec != QF_PREC_F64) return fail(QF_ERR_INVALID, "prec %d", prec);
     if (by != 0 && (by % 2 || by < 2 || by > 6)) return fail(QF_ERR_INVALID, "patch height %d not in {0,2,4,6}", by);
-    if (t < 0 || (tb != 0 && !tb_ok(tb))) return fail(QF_ERR_INVALID, "bad tile size (t=%d, tb=%d not in {0,16,32,64})", t, tb);
+    if (t < 0 || (tb != 0 && !tb_ok(tb))) return fail(QF_ERR_INVALID, "bad tile size (t=%d, tb=%d not in {0,16,32,64,128})", t, tb);
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune[prec].by = by;
     g_tune[prec].t = t;
@@ -741,7 +744,7 @@ int qf_precon_vec(int prec, const void* ptab, const void* r, void* z, int nrhs, 
     if (Vs < 1 || Ltau < 1 || nrhs < 1 || nrhs > 65535) return fail(QF_ERR_INVALID, "bad sizes");
     hipStream_t s = (hipStream_t)stream;
     Tiling t{};
-    t.TB = Ltau <= 16 ? 16 : 32;
+    t.TB = Ltau >= 128 ? 128 : Ltau >= 64 ? 64 : Ltau > 16 ? 32 : 16;
     t.nsx = (Vs + cgu_tx(t.TB) - 1) / cgu_tx(t.TB);
     auto run = [&](auto tag) -> int {
         using R = decltype(tag);
@@ -823,7 +826,7 @@ int qf_ragged_plan(int nprob, const int* Ls, const int* Lts, void* plan, size_t 
     RaggedHead* h = static_cast<RaggedHead*>(plan);
     Prob* P = reinterpret_cast<Prob*>(h + 1);
     std::memset(plan, 0, qf_ragged_plan_bytes(nprob));
-    h->nprob = nprob; h->by = 2; h->tb = 32;
+    h->nprob = nprob; h->by = 2; h->tb = 64;
     int maxthr = 0;
     long long voff = 0, toff = 0, wg = 0, bwg = 0, pa = 0, pb = 0;
     for (int k = 0; k < nprob; ++k) {

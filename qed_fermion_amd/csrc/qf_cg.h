@@ -45,15 +45,19 @@ struct CguArgs {
     int nprob;
 };
 
-// One thread per (site, slice) of a TX x TT tile; the new residual of the
-// tile's TT + 12 slices (the 13-point tau window) is staged in LDS, so no
-// thread marches in tau and all loads of a block are in flight at once.
-// Slices outside [0, Lt) of the window wrap cyclically (P is cyclic).
+// A block of TX * TT / SPT threads owns a tile of TX sites x TT slices; each
+// thread owns SPT slices (rows sr, sr + RT, ...) of one site.  The new
+// residual of the tile's TT + 12 slices (the 13-point tau window) is staged in
+// LDS, so no thread marches in tau and all loads of a block are in flight at
+// once.  Slices outside [0, Lt) of the window wrap cyclically (P is cyclic).
 // Partial sums: one (rz, rr) pair per block, blk = blockIdx.x + nsx * blockIdx.y.
-template <typename R, int MODE, bool PRE, int TX, int TT>
-__global__ void __launch_bounds__(TX * TT) cg_update_kernel(CguArgs<R> A) {
+template <typename R, int MODE, bool PRE, int TX, int TT, int SPT>
+__global__ void __launch_bounds__(TX * TT / SPT) cg_update_kernel(CguArgs<R> A) {
     constexpr int H = PRE ? 12 : 0;   // halo slices
     constexpr int ROWS = TT + H;
+    constexpr int RT = TT / SPT;      // thread rows
+    constexpr int NLD = (ROWS + RT - 1) / RT;
+    static_assert(TT % SPT == 0, "slices per thread must divide the tile");
     __shared__ cplx<R> ring[PRE ? ROWS * TX : 1];
     __shared__ double red[64];
     int rho, Vs, Lt, bx, by, blk, nA;
@@ -73,12 +77,10 @@ __global__ void __launch_bounds__(TX * TT) cg_update_kernel(CguArgs<R> A) {
         blk = bx + gridDim.x * by;
         vo = rho * A.vs; pout = (long long)rho * gridDim.x * gridDim.y + blk; pin = (long long)rho * A.nA; nA = A.nA;
     }
-    const int sx = threadIdx.x % TX, st = threadIdx.x / TX;
+    const int sx = threadIdx.x % TX, sr = threadIdx.x / TX;
     const int s = bx * TX + sx;
     const int t0 = by * TT;
-    const int tau = t0 + st;
     const bool col = s < Vs;
-    const bool own = col && tau < Lt;
     const int sc = col ? s : 0;   // loads of padding columns read site 0 (never stored)
 
     if constexpr (MODE == CGU_ITER) {
@@ -97,22 +99,37 @@ __global__ void __launch_bounds__(TX * TT) cg_update_kernel(CguArgs<R> A) {
     // ---- loads (none depends on the step size)
     const cplx<R>* rold = A.rold + vo;
     const cplx<R>* op = A.op + vo;
-    // window rows h = st and h = st + TT (if < ROWS): slice (t0 - H/2 + h) mod Lt
     auto slice_of = [&](int h) { return ((t0 - H / 2 + h) % Lt + Lt) % Lt; };
-    const int ta = PRE ? slice_of(st) : min(tau, Lt - 1);
-    const bool second = PRE && st + TT < ROWS;
-    const int tb = second ? slice_of(st + TT) : ta;
-    cplx<R> ra = rold[(long long)ta * Vs + sc], rb = mk<R>(0, 0), oa = mk<R>(0, 0), ob = mk<R>(0, 0);
-    if (second) rb = rold[(long long)tb * Vs + sc];
-    if constexpr (MODE == CGU_ITER) {
-        oa = op[(long long)ta * Vs + sc];
-        if (second) ob = op[(long long)tb * Vs + sc];
+    cplx<R> rw[PRE ? NLD : SPT], ow[PRE ? NLD : SPT];
+    if constexpr (PRE) {
+#pragma unroll
+        for (int k = 0; k < NLD; ++k) {
+            const int h = sr + k * RT;
+            const long long i = (long long)slice_of(h < ROWS ? h : 0) * Vs + sc;
+            rw[k] = mk<R>(0, 0); ow[k] = mk<R>(0, 0);
+            if (h < ROWS) {
+                rw[k] = rold[i];
+                if constexpr (MODE == CGU_ITER) ow[k] = op[i];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < SPT; ++j) {
+            const long long i = (long long)min(t0 + sr + j * RT, Lt - 1) * Vs + sc;
+            rw[j] = rold[i];
+            ow[j] = mk<R>(0, 0);
+            if constexpr (MODE == CGU_ITER) ow[j] = op[i];
+        }
     }
-    const long long io = vo + (long long)min(tau, Lt - 1) * Vs + sc;   // own element (clamped for loads)
-    cplx<R> pp = mk<R>(0, 0), xx = mk<R>(0, 0);
-    if constexpr (MODE == CGU_ITER) {
-        pp = A.p[io];
-        xx = A.x[io];
+    cplx<R> pp[SPT], xx[SPT];
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+        pp[j] = xx[j] = mk<R>(0, 0);
+        if constexpr (MODE == CGU_ITER) {
+            const long long io = vo + (long long)min(t0 + sr + j * RT, Lt - 1) * Vs + sc;
+            pp[j] = A.p[io];
+            xx[j] = A.x[io];
+        }
     }
 
     R alpha = R(0);
@@ -121,36 +138,46 @@ __global__ void __launch_bounds__(TX * TT) cg_update_kernel(CguArgs<R> A) {
         const double* src[1] = {A.pap + pin};
         sum_partials<1>(src, nA, sums, red);
         alpha = (R)(A.st[rho].rz / sums[0]);
-        ra = mk<R>(ra.x - alpha * oa.x, ra.y - alpha * oa.y);
-        rb = mk<R>(rb.x - alpha * ob.x, rb.y - alpha * ob.y);
-    }
-
-    cplx<R> r, zz;
-    if constexpr (PRE) {
-        ring[st * TX + sx] = ra;
-        if (second) ring[(st + TT) * TX + sx] = rb;
-        __syncthreads();
-        r = ring[(st + H / 2) * TX + sx];
-        const R* c = ptab + min(tau, Lt - 1) * 13;
-        zz = mk<R>(0, 0);
 #pragma unroll
-        for (int k = 0; k < 13; ++k) {
-            const cplx<R> w = ring[(st + k) * TX + sx];
-            zz = mk<R>(zz.x + c[k] * w.x, zz.y + c[k] * w.y);
+        for (int k = 0; k < (PRE ? NLD : SPT); ++k) rw[k] = mk<R>(rw[k].x - alpha * ow[k].x, rw[k].y - alpha * ow[k].y);
+    }
+    if constexpr (PRE) {
+#pragma unroll
+        for (int k = 0; k < NLD; ++k) {
+            const int h = sr + k * RT;
+            if (h < ROWS) ring[h * TX + sx] = rw[k];
         }
-    } else {
-        r = ra;
-        zz = r;
+        __syncthreads();
     }
 
     double rr = 0.0, rz = 0.0;
-    if (own) {
-        if constexpr (MODE == CGU_ITER) A.x[io] = mk<R>(xx.x + alpha * pp.x, xx.y + alpha * pp.y);
-        if constexpr (MODE == CGU_INIT) A.x[io] = mk<R>(0, 0);
-        if constexpr (MODE == CGU_INIT || MODE == CGU_ITER) A.rnew[io] = r;
-        A.z[io] = zz;
-        rr = (double)r.x * r.x + (double)r.y * r.y;
-        rz = (double)r.x * zz.x + (double)r.y * zz.y;
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+        const int st = sr + j * RT;
+        const int tau = t0 + st;
+        cplx<R> r, zz;
+        if constexpr (PRE) {
+            r = ring[(st + H / 2) * TX + sx];
+            const R* c = ptab + min(tau, Lt - 1) * 13;
+            zz = mk<R>(0, 0);
+#pragma unroll
+            for (int k = 0; k < 13; ++k) {
+                const cplx<R> w = ring[(st + k) * TX + sx];
+                zz = mk<R>(zz.x + c[k] * w.x, zz.y + c[k] * w.y);
+            }
+        } else {
+            r = rw[j];
+            zz = r;
+        }
+        if (col && tau < Lt) {
+            const long long io = vo + (long long)tau * Vs + s;
+            if constexpr (MODE == CGU_ITER) A.x[io] = mk<R>(xx[j].x + alpha * pp[j].x, xx[j].y + alpha * pp[j].y);
+            if constexpr (MODE == CGU_INIT) A.x[io] = mk<R>(0, 0);
+            if constexpr (MODE == CGU_INIT || MODE == CGU_ITER) A.rnew[io] = r;
+            A.z[io] = zz;
+            rr += (double)r.x * r.x + (double)r.y * r.y;
+            rz += (double)r.x * zz.x + (double)r.y * zz.y;
+        }
     }
     if constexpr (MODE != CGU_PRECON) {
         double v[2] = {rz, rr};
