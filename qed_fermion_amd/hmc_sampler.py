@@ -26,7 +26,7 @@ import os
 
 import torch
 
-from . import kernels, precon
+from . import checkpoint, kernels, precon
 from .trajectory import TrajectoryEngine, couplings
 
 
@@ -39,7 +39,7 @@ class HmcSampler(object):
 
     def __init__(self, Lx=6, Ltau=10, J=0.5, Nstep=3000, config=None, *, bs=None, dtau=None, K=None, compact=None,
                  precon_on=None, max_iter=None, cg_rtol=1e-9, cuda_graph=None, precision="fp32", ru_period=50,
-                 device="cuda", seed=None):
+                 device="cuda", seed=None, ckpt_root=None):
         self.Lx = Lx
         self.Ly = Lx
         self.Ltau = Ltau
@@ -87,14 +87,37 @@ class HmcSampler(object):
         self.leapfrog_cmp_graph_runners = None
         self.graph_memory_pool = None
         self._engine = None
+        self.step = 0
         self.cur_step = 0
+        # Folder holding check_points/ and preconditioners/ (the reference keeps them next to its
+        # sources); None: start from the pi-flux configuration and build P in memory.
+        self.ckpt_root = ckpt_root
         self.reset()
 
     # ------------------------------------------------------------------ setup
     def reset(self):
+        """reset (hmc_sampler_batch.py:321-328): geometry, run tag, then the ensemble start
+        when a checkpoint root is set (pi flux plus noise otherwise)."""
         self.Vs = self.Lx * self.Ly
         self.initialize_geometry()
-        self.initialize_boson_pi_flux_randn_matfree()
+        self.initialize_specifics()
+        if self.ckpt_root is not None:
+            self.initialize_boson_ensemble()
+        else:
+            self.initialize_boson_pi_flux_randn_matfree()
+
+    def initialize_specifics(self):
+        """The run tag used in file names (hmc_sampler_batch.py:330-331)."""
+        self.specifics = checkpoint.specifics(self)
+
+    def initialize_boson_ensemble(self, root=None):
+        """Start from a saved ensemble member under ``root``/check_points/boson_ensemble
+        (hmc_sampler_batch.py:971-1016); falls back to the pi-flux start.  Returns the members found."""
+        return checkpoint.initialize_boson_ensemble(self, root if root is not None else self.ckpt_root)
+
+    def save_to_file(self, res, data_folder, filename):
+        """save_to_file (hmc_sampler_batch.py:3211-3215)."""
+        return checkpoint.save_to_file(res, data_folder, filename)
 
     def initialize_geometry(self):
         """Checkerboard bond groups (hmc_sampler_batch.py:937-969): i/j lists and boson indices."""
@@ -131,10 +154,23 @@ class HmcSampler(object):
 
     def reset_precon(self):
         """reset_precon (hmc_sampler_batch.py:461-547): the manual preconditioner as precon_csr
-        (complex64 CSR, like the reference) and as the band table the kernels apply."""
-        d = precon.get_precon_man(self.Lx, self.Ltau)
-        self.precon_csr = precon.coo_to_csr(d, device=self.device)
-        self.precon_table = precon.band_table(self.Ltau, dtype=torch.float64, device=self.device)
+        (complex64 CSR, like the reference) and as the band table the kernels apply.
+
+        With a checkpoint root, P comes from ``preconditioners/precon_ckpt_L_..._t_1.pt`` there
+        (verified to be the tau-banded P_tau (x) I); a missing file is written from the manual
+        preconditioner first, as the reference does through get_precon_man."""
+        if self.ckpt_root is None:
+            d = precon.get_precon_man(self.Lx, self.Ltau)
+            self.precon_csr = precon.coo_to_csr(d, device=self.device)
+            self.precon_table = precon.band_table(self.Ltau, dtype=torch.float64, device=self.device)
+            return
+        path = os.path.join(self.ckpt_root, "preconditioners",
+                            checkpoint.precon_file_name(self.Lx, self.Ltau, self.dtau, self.t))
+        if not os.path.exists(path):
+            checkpoint.save_precon(precon.get_precon_man(self.Lx, self.Ltau), path)
+        tab, csr = checkpoint.precon_table_from_file(path, self.Lx, self.Ltau)
+        self.precon_csr = csr.to(self.device)
+        self.precon_table = tab.to(self.device)
 
     def _ptab(self):
         if not (self.dtau <= 0.1 and self.precon_on):
@@ -289,7 +325,6 @@ class HmcSampler(object):
         thresholds = torch.minimum(torch.exp(H_old - H_new), torch.ones_like(H_old))
         for b in range(self.bs):
             self.threshold_queue[b].append(thresholds[b].item())
-        self.cur_step += 1
         return self.boson, accp, cg_converge_iter, cg_r_err
 
     def adjust_delta_t(self):
