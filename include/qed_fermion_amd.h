@@ -125,9 +125,8 @@ int qf_pcg_mixed_solve(const double* phi64, const float* phi32, long long phi_cs
                        size_t work_bytes, double* err_out, double* iters_out, void* stream);
 
 /* Ragged batch: nprob independent lattices of different L (one chain each)
- * in the same launches -- the batching of several lattices of uneven size
- * that BASELINE's batching configuration asks for; no counterpart in the
- * reference, whose batches share one L.  Data are concatenated per lattice
+ * in the same launches (for example L in {12, 18, 24, 30} with Ltau = 10 L);
+ * no counterpart in the reference, whose batches share one L.  Data are concatenated per lattice
  * (vectors: Ltau*L*L complex each; phi: 2*Ltau*L*L reals; preconditioner
  * tables: 13*Ltau each).  qf_ragged_plan fills a host plan of
  * qf_ragged_plan_bytes(nprob) bytes; the caller keeps an identical copy in
