@@ -138,15 +138,25 @@ class HmcSampler(object):
         self.boson_idx_list_3 = self.i_list_3 * 2
         self.boson_idx_list_4 = self.i_list_4 * 2 + 1
 
-    def initialize_boson_pi_flux_randn_matfree(self, sigma=0.5):
-        """Staggered pi flux plus Gaussian noise (the reference's fallback ensemble, 1026-1043)."""
+    def initialize_boson_pi_flux_randn_matfree(self):
+        """Staggered pi flux (x-link sign*pi/4, y-link -sign*pi/4, sign = (-1)^(x+y)) for the last
+        chain; the other bs-1 chains add tau-independent Gaussian noise with standard deviations
+        linspace(0.02, 0.08, bs-1) (hmc_sampler_batch.py:1026-1043)."""
         x = torch.arange(self.Lx, device=self.device).view(-1, 1)
         y = torch.arange(self.Ly, device=self.device).view(1, -1)
         sign = (1 - 2 * ((x + y) % 2)).to(self.dtype)
-        b = torch.zeros(self.bs, 2, self.Lx, self.Ly, self.Ltau, device=self.device, dtype=self.dtype)
-        b[:, 0] = (sign * math.pi / 4).unsqueeze(-1)
-        b[:, 1] = (-sign * math.pi / 4).unsqueeze(-1)
-        self.boson = b + sigma * torch.randn_like(b)
+        stag = torch.zeros(1, 2, self.Lx, self.Ly, self.Ltau, device=self.device, dtype=self.dtype)
+        stag[:, 0] = (sign * math.pi / 4).unsqueeze(-1)
+        stag[:, 1] = (-sign * math.pi / 4).unsqueeze(-1)
+        noise = torch.randn(self.bs - 1, 2, self.Lx, self.Ly, 1, device=self.device, dtype=self.dtype)
+        levels = torch.linspace(0.02, 0.08, self.bs - 1, device=self.device, dtype=self.dtype).view(-1, 1, 1, 1, 1)
+        self.boson = torch.cat([stag.repeat(self.bs - 1, 1, 1, 1, 1) + noise * levels, stag], dim=0)
+
+    def initialize_boson_test(self):
+        """Zero flux plus Gaussian noise of standard deviation linspace(0.1, 0.5, bs) per chain
+        (hmc_sampler_batch.py:1018-1024)."""
+        levels = torch.linspace(0.1, 0.5, self.bs, device=self.device, dtype=self.dtype).view(-1, 1, 1, 1, 1)
+        self.boson = torch.randn(self.bs, 2, self.Lx, self.Ly, self.Ltau, device=self.device, dtype=self.dtype) * levels
 
     def initialize_boson_uniform(self):
         self.boson = (torch.rand(self.bs, 2, self.Lx, self.Ly, self.Ltau, device=self.device, dtype=torch.float64) * 2 - 1) \

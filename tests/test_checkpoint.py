@@ -6,6 +6,7 @@ rule at 58-62).  Those runs predate the ``max_iter`` field, which the current
 tag carries between cg_rtol and max_block_idx, so the comparison takes it out.
 """
 import collections
+import math
 import os
 import pickle
 
@@ -181,3 +182,18 @@ def test_loads_execute_nothing(tmp_path):
         checkpoint.load_precon(path)
     with pytest.raises(pickle.UnpicklingError):
         checkpoint.load_checkpoint(path)
+
+
+def test_pi_flux_start_matches_reference_recipe():
+    # hmc_sampler_batch.py:1026-1043: the last chain is the staggered pi flux, the others add
+    # tau-independent noise of standard deviation 0.02 ... 0.08
+    torch.manual_seed(3)
+    s = HmcSampler(Lx=8, Ltau=40, bs=4, device="cpu")
+    b = s.boson
+    sign = torch.tensor([[(-1) ** (x + y) for y in range(8)] for x in range(8)], dtype=b.dtype)
+    stag = torch.stack([sign * math.pi / 4, -sign * math.pi / 4]).unsqueeze(-1).expand(2, 8, 8, 40)
+    assert torch.equal(b[-1], stag)
+    dev = b[:-1] - stag
+    assert torch.equal(dev, dev[..., :1].expand_as(dev))           # constant along tau
+    sd = dev[..., 0].reshape(3, -1).std(dim=1)
+    assert (sd[0] < sd[1] < sd[2]) and 0.01 < sd[0] < 0.04 and 0.05 < sd[2] < 0.12
