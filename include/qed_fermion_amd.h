@@ -168,6 +168,42 @@ size_t qf_hamiltonian_workspace_bytes(int nrhs);
 int qf_hamiltonian(int prec, const double* x, const double* p, const void* psi, const void* xi, double* H, double* parts, int nrhs, int L,
                    int Ltau, double cJ, double cK, int compact, double m, void* work, size_t work_bytes, void* stream);
 
+/* Bond-bond stochastic estimators (stochastic_estimator.py:1393-1654, T1 / T21 /
+ * T2 / T4 pair estimators and Tvv; combined by get_bond_bond_per_b2, 2594-2641).
+ * Each is a correlation C(Delta) = (1/V) sum_r a(r - Delta) b(r) of fields a, b
+ * built from two noise vectors, averaged over pairs s < s' and tau slices.
+ * The reference computes it per pair batch with three FFTs (1428-1430); here
+ * the caller 2-D-transforms the fields of a chunk of pairs (any FFT library,
+ * e.g. rocFFT), qf_obs_kspace sums Ahat(k) Bhat(-k) over every slice in FP64
+ * into per-work-group partials, and one transform of the reduced [L, L]
+ * array gives C.  eta and G (= M^{-1} eta) are [nrv][Ltau][L][L] complex
+ * (complex64 for QF_PREC_F32, complex128 for QF_PREC_F64); phix [Ltau][L][L]
+ * is the chain's x-link angle in the matching real type. */
+#define QF_OBS_T1 0
+#define QF_OBS_T21 1
+#define QF_OBS_T2 2
+#define QF_OBS_T4 3
+#define QF_OBS_V 4
+/* Fields a -> A[j], b -> B[j] for pairs j = p0 .. p0+np-1 of pairs[2*j], pairs[2*j+1]
+ * (s < s', torch.combinations order; entries outside [0, nrv) give zero fields).
+ * QF_OBS_V writes v_s of Tvv for noise vectors p0 .. p0+np-1 into A (pairs, B unused).
+ * A, B: [np][Ltau*L*L]. */
+int qf_obs_fields(int prec, int obs, const void* eta, const void* G, const void* phix, const int* pairs, int p0, int np, int nrv, int L,
+                  int Ltau, void* A, void* B, void* stream);
+/* part[w][k] (+)= sum over slices sl = w, w+nwg, ... < nslices of A_hat[sl][k] * B_hat[sl][-k]
+ * (double2, [nwg][L*L]); with lin also part[nwg + w][k] (+)= sum A_hat[sl][k].
+ * accumulate = 0 overwrites, so the first chunk needs no cleared buffer. */
+int qf_obs_kspace(int prec, const void* A_hat, const void* B_hat, long long nslices, int L, int lin, int accumulate, void* part, int nwg,
+                  void* stream);
+/* out[k] = sum_{w < nw} part[w*n + k] (double2), in order. */
+int qf_obs_reduce(const void* part, int nw, int n, void* out, void* stream);
+/* Same-noise-vector sums the estimators add at one Delta each, over all nrv
+ * vectors and sites, as part[block][3] double2 (nblk blocks; reduce with
+ * qf_obs_reduce(part, nblk, 3, ...)): 0: G(r+x) e(r-x) U(r) U(r-x) (T1),
+ * 1: G(r) e(r) (T21, T2), 2: G(r) e(r+2x) conj(U(r) U(r+x)) (T4); e = conj(eta),
+ * U = exp(i phix). */
+int qf_obs_diag(int prec, const void* eta, const void* G, const void* phix, int nrv, int L, int Ltau, void* part, int nblk, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

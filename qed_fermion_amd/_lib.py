@@ -26,6 +26,12 @@ QF_OP_MDAG = 2
 QF_OP_B = 3
 QF_OP_MDAGM = 4
 
+QF_OBS_T1 = 0
+QF_OBS_T21 = 1
+QF_OBS_T2 = 2
+QF_OBS_T4 = 3
+QF_OBS_V = 4
+
 _ERR_NAMES = {1: "QF_ERR_INVALID", 2: "QF_ERR_ODD_L", 3: "QF_ERR_LTAU", 4: "QF_ERR_HIP",
               5: "QF_ERR_UNSUPPORTED", 6: "QF_ERR_PRECON"}
 
@@ -72,6 +78,10 @@ SIGNATURES = {
     "qf_momentum_kick": (_I, [_P, _P, _P, _D, _P, _I, _I, _I, _P]),
     "qf_hamiltonian_workspace_bytes": (_SZ, [_I]),
     "qf_hamiltonian": (_I, [_I, _P, _P, _P, _P, _P, _P, _I, _I, _I, _D, _D, _I, _D, _P, _SZ, _P]),
+    "qf_obs_fields": (_I, [_I, _I, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _P, _P]),
+    "qf_obs_kspace": (_I, [_I, _P, _P, _LL, _I, _I, _I, _P, _I, _P]),
+    "qf_obs_reduce": (_I, [_P, _I, _I, _P, _P]),
+    "qf_obs_diag": (_I, [_I, _P, _P, _P, _I, _I, _I, _P, _I, _P]),
 }
 
 _lib = None

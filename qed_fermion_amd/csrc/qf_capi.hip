@@ -21,6 +21,7 @@
 #include "qf_boson.h"
 #include "qf_cg.h"
 #include "qf_force.h"
+#include "qf_obs.h"
 #include "qf_operator.h"
 
 using namespace qf;
@@ -1070,4 +1071,79 @@ int qf_hamiltonian(int prec, const double* x, const double* p, const void* psi, 
     return QF_OK;
 }
 
+// ---------------------------------------------------------------- bond-bond estimators
+int qf_obs_fields(int prec, int obs, const void* eta, const void* G, const void* phix, const int* pairs, int p0, int np, int nrv, int L,
+                  int Ltau, void* A, void* B, void* stream) {
+    if (!eta || !G || !phix || !A) return fail(QF_ERR_INVALID, "null pointer");
+    if (obs < QF_OBS_T1 || obs > QF_OBS_V) return fail(QF_ERR_INVALID, "estimator %d", obs);
+    if (obs != QF_OBS_V && (!pairs || !B)) return fail(QF_ERR_INVALID, "pair estimators need pairs and B");
+    if (L < 2 || Ltau < 1 || nrv < 1 || np < 1 || p0 < 0) return fail(QF_ERR_INVALID, "bad sizes");
+    if (obs == QF_OBS_V && p0 + np > nrv) return fail(QF_ERR_INVALID, "noise vectors [%d, %d) beyond nrv %d", p0, p0 + np, nrv);
+    const long long N = (long long)L * L * Ltau, total = (long long)np * N;
+    const dim3 grid((unsigned)std::min<long long>((total + 255) / 256, 32768)), blk(256);
+    hipStream_t s = (hipStream_t)stream;
+    auto run = [&](auto tag) -> int {
+        using R = decltype(tag);
+        ObsArgs<R> a{};
+        a.eta = (const cplx<R>*)eta; a.G = (const cplx<R>*)G; a.phix = (const R*)phix; a.pairs = pairs;
+        a.p0 = p0; a.np = np; a.nrv = nrv; a.L = L; a.N = N; a.A = (cplx<R>*)A; a.B = (cplx<R>*)B;
+        switch (obs) {
+            case QF_OBS_T1: hipLaunchKernelGGL((obs_fields_kernel<R, OBS_T1>), grid, blk, 0, s, a); break;
+            case QF_OBS_T21: hipLaunchKernelGGL((obs_fields_kernel<R, OBS_T21>), grid, blk, 0, s, a); break;
+            case QF_OBS_T2: hipLaunchKernelGGL((obs_fields_kernel<R, OBS_T2>), grid, blk, 0, s, a); break;
+            case QF_OBS_T4: hipLaunchKernelGGL((obs_fields_kernel<R, OBS_T4>), grid, blk, 0, s, a); break;
+            default: hipLaunchKernelGGL((obs_fields_kernel<R, OBS_V>), grid, blk, 0, s, a); break;
+        }
+        QF_LAUNCHED();
+        return QF_OK;
+    };
+    if (prec == QF_PREC_F32) return run(float{});
+    if (prec == QF_PREC_F64) return run(double{});
+    return fail(QF_ERR_INVALID, "prec %d", prec);
+}
+
+int qf_obs_kspace(int prec, const void* A_hat, const void* B_hat, long long nslices, int L, int lin, int accumulate, void* part, int nwg,
+                  void* stream) {
+    if (!A_hat || !B_hat || !part) return fail(QF_ERR_INVALID, "null pointer");
+    if (L < 2 || nslices < 1 || nwg < 1 || nwg > 65535) return fail(QF_ERR_INVALID, "bad sizes");
+    hipStream_t s = (hipStream_t)stream;
+    if (prec == QF_PREC_F32)
+        hipLaunchKernelGGL((obs_kspace_kernel<float>), dim3(nwg), dim3(256), 0, s, (const float2*)A_hat, (const float2*)B_hat, nslices, L,
+                           lin, accumulate, (double2*)part, nwg);
+    else if (prec == QF_PREC_F64)
+        hipLaunchKernelGGL((obs_kspace_kernel<double>), dim3(nwg), dim3(256), 0, s, (const double2*)A_hat, (const double2*)B_hat, nslices,
+                           L, lin, accumulate, (double2*)part, nwg);
+    else
+        return fail(QF_ERR_INVALID, "prec %d", prec);
+    QF_LAUNCHED();
+    return QF_OK;
+}
+
+int qf_obs_reduce(const void* part, int nw, int n, void* out, void* stream) {
+    if (!part || !out) return fail(QF_ERR_INVALID, "null pointer");
+    if (nw < 1 || n < 1) return fail(QF_ERR_INVALID, "bad sizes");
+    hipLaunchKernelGGL(obs_reduce_kernel, dim3((unsigned)std::min(((long long)n + 255) / 256, 4096LL)), dim3(256), 0, (hipStream_t)stream,
+                       (const double2*)part, nw, n, (double2*)out);
+    QF_LAUNCHED();
+    return QF_OK;
+}
+
+int qf_obs_diag(int prec, const void* eta, const void* G, const void* phix, int nrv, int L, int Ltau, void* part, int nblk, void* stream) {
+    if (!eta || !G || !phix || !part) return fail(QF_ERR_INVALID, "null pointer");
+    if (L < 2 || Ltau < 1 || nrv < 1 || nblk < 1 || nblk > 65535) return fail(QF_ERR_INVALID, "bad sizes");
+    const long long N = (long long)L * L * Ltau;
+    hipStream_t s = (hipStream_t)stream;
+    if (prec == QF_PREC_F32)
+        hipLaunchKernelGGL((obs_diag_kernel<float>), dim3(nblk), dim3(256), 0, s, (const float2*)eta, (const float2*)G, (const float*)phix, nrv,
+                           L, N, (double2*)part);
+    else if (prec == QF_PREC_F64)
+        hipLaunchKernelGGL((obs_diag_kernel<double>), dim3(nblk), dim3(256), 0, s, (const double2*)eta, (const double2*)G,
+                           (const double*)phix, nrv, L, N, (double2*)part);
+    else
+        return fail(QF_ERR_INVALID, "prec %d", prec);
+    QF_LAUNCHED();
+    return QF_OK;
+}
+
 }  // extern "C"
+

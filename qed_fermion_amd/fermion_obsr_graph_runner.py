@@ -1,9 +1,11 @@
-"""GEtaGraphRunner: capture-once / replay of StochaticEstimator.set_eta_G_eta_inner
-(qed_fermion/fermion_obsr_graph_runner.py:110-191), same API.
+"""Capture-once / replay runners of qed_fermion/fermion_obsr_graph_runner.py, same API.
 
-capture(boson, eta, graph_memory_pool=None, n_warmups=3) -> pool;  __call__(boson, eta) -> G_eta
-The observable runners of that file (T1..Tvv, spsm) wrap FFT contractions that
-are outside this package.
+GEtaGraphRunner (110-191): capture(boson, eta, graph_memory_pool=None, n_warmups=3) -> pool;
+    __call__(boson, eta) -> G_eta
+T1/T21/T2/T4/TvvGraphRunner (365-722): capture(graph_memory_pool=None, n_warmups=3) -> pool;
+    __call__(eta, G_eta, boson) -> the estimator's output (Tvv: (Tvv, Tv))
+The estimator graphs hold the HIP field / k-space kernels and the rocFFT
+transforms between them; inputs are copied into static buffers before each replay.
 """
 import torch
 
@@ -37,3 +39,67 @@ class GEtaGraphRunner:
         self.input_buffers["eta"].copy_(eta)
         self.graph.replay()
         return self.output_buffers["G_eta"]
+
+
+class _EstimatorGraphRunner:
+    method = None
+
+    def __init__(self, se):
+        self.se = se
+        self.graph = None
+        self.input_buffers = {}
+        self.output_buffers = None
+
+    def capture(self, graph_memory_pool=None, n_warmups=3):
+        se = self.se
+        n = se.Ltau * se.Ly * se.Lx
+        ib = {"eta": torch.zeros((se.Nrv, n), device=se.device, dtype=se.cdtype),
+              "G_eta": torch.zeros((se.Nrv, n), device=se.device, dtype=se.cdtype),
+              "boson": torch.zeros((1, 2, se.Lx, se.Ly, se.Ltau), device=se.device, dtype=se.dtype)}
+        fn = getattr(se, self.method)
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            for _ in range(n_warmups):
+                fn(ib["eta"], ib["G_eta"], ib["boson"])
+            s.synchronize()
+        torch.cuda.current_stream().wait_stream(s)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, pool=graph_memory_pool):
+            out = fn(ib["eta"], ib["G_eta"], ib["boson"])
+        self.graph = graph
+        self.input_buffers = ib
+        self.output_buffers = out
+        return graph.pool()
+
+    def __call__(self, eta, G_eta, boson):
+        self.input_buffers["eta"].copy_(eta)
+        self.input_buffers["G_eta"].copy_(G_eta)
+        self.input_buffers["boson"].copy_(boson)
+        self.graph.replay()
+        return self.output_buffers
+
+
+class T1GraphRunner(_EstimatorGraphRunner):
+    """fermion_obsr_graph_runner.py:365-434."""
+    method = "T1"
+
+
+class T21GraphRunner(_EstimatorGraphRunner):
+    """fermion_obsr_graph_runner.py:436-505."""
+    method = "T21"
+
+
+class T2GraphRunner(_EstimatorGraphRunner):
+    """fermion_obsr_graph_runner.py:507-576."""
+    method = "T2"
+
+
+class T4GraphRunner(_EstimatorGraphRunner):
+    """fermion_obsr_graph_runner.py:578-647."""
+    method = "T4"
+
+
+class TvvGraphRunner(_EstimatorGraphRunner):
+    """fermion_obsr_graph_runner.py:649-720."""
+    method = "Tvv"

@@ -5,20 +5,27 @@ Same class and method names (the reference's spelling), same tensor shapes:
   random_vec_bin()          -> eta [Nrv, Ltau*Ly*Lx], entries (+-1 +- i)/sqrt(2)      (210-224)
   set_eta_G_eta(boson, eta) -> self.G_eta = (M^dag M)^{-1} M^dag eta = M^{-1} eta      (409-433)
 
-MI355X difference: the reference repeats the boson field Nrv times
+  T1 / T21 / T2 / T4 / Tvv(eta, G_eta, boson)   bond-bond estimators    (1393-1654)
+  get_bond_bond_per_b2(boson), get_fermion_obsr_compile(bosons, eta)    (2492-2641)
+  bond_corr, reorder_fft_grid2[_inverse]                                (253-262, 2643-2651)
+
+MI355X differences: the reference repeats the boson field Nrv times
 (``boson.repeat(Nrv, 1)``, 429) so its batched kernels see one field per RHS;
 here all Nrv right-hand sides share ONE field (phi stride 0), so M^dag and
 every PCG iteration read phi / its hop table once per launch for the whole
-batch, in one multi-RHS launch per iteration.  The FFT observables built on
-G_eta (T1..Tvv, 1393-1651) are outside this package.
+batch, in one multi-RHS launch per iteration.  The bond-bond estimators run
+on HIP kernels around rocFFT transforms (qed_fermion_amd/observables.py):
+chunks of pairs instead of batches of int(0.1 Nrv), k-space products summed
+over all pairs and slices in FP64, one inverse transform per estimator.
 """
 import math
 import os
 
 import torch
 
-from . import kernels
-from .fermion_obsr_graph_runner import GEtaGraphRunner
+from . import kernels, observables
+from .fermion_obsr_graph_runner import (GEtaGraphRunner, T1GraphRunner, T2GraphRunner, T4GraphRunner, T21GraphRunner,
+                                        TvvGraphRunner)
 
 
 class StochaticEstimator:
@@ -35,6 +42,12 @@ class StochaticEstimator:
         self.cdtype = hmc.cdtype
         self.graph_memory_pool = hmc.graph_memory_pool
         self.G_eta_graph_runner = GEtaGraphRunner(self)
+        self.T1_graph_runner = T1GraphRunner(self)
+        self.T21_graph_runner = T21GraphRunner(self)
+        self.T2_graph_runner = T2GraphRunner(self)
+        self.T4_graph_runner = T4GraphRunner(self)
+        self.Tvv_graph_runner = TvvGraphRunner(self)
+        self._bb_est = None
         self.num_samples = lambda nrv: math.comb(nrv, 2)
         self.batch_size = lambda nrv: int(nrv * 0.1)
         self.G_eta = None
@@ -46,10 +59,14 @@ class StochaticEstimator:
         self.indices_r2 = torch.combinations(torch.arange(self.Nrv, device=self.device), r=2, with_replacement=False)
 
     def init_cuda_graph(self):
+        """Capture the G_eta solve and the five estimators (stochastic_estimator.py:113-200)."""
         if self.cuda_graph_se:
             dummy_eta = torch.zeros((self.Nrv, self.Ltau * self.Vs), device=self.device, dtype=self.cdtype)
             dummy_boson = torch.zeros(1, 2, self.Lx, self.Ly, self.Ltau, device=self.device, dtype=self.dtype)
             self.graph_memory_pool = self.G_eta_graph_runner.capture(dummy_boson, dummy_eta, self.graph_memory_pool)
+            for r in (self.T1_graph_runner, self.T21_graph_runner, self.T2_graph_runner, self.T4_graph_runner,
+                      self.Tvv_graph_runner):
+                self.graph_memory_pool = r.capture(self.graph_memory_pool)
 
     def random_vec_bin(self):
         """Nrv Z2 x Z2 noise vectors (+-1 +- i)/sqrt(2), [Nrv, Lx*Ly*Ltau] (stochastic_estimator.py:210-224)."""
@@ -84,3 +101,57 @@ class StochaticEstimator:
             G_eta, _, _ = kernels.pcg_solve(phi, psi, self.Lx, self.Ltau, self.dtau, ptab=ptab, rtol=hmc.cg_rtol,
                                             max_iter=self.max_iter_se)
         return G_eta
+
+    # ---------------------------------------------------------- bond-bond estimators
+    def _bb(self, eta):
+        est = self._bb_est
+        if est is None or est.dtype != eta.dtype or est.nrv != eta.shape[0]:
+            est = observables.BondBondEstimator(self.Lx, self.Ltau, eta.shape[0], dtype=eta.dtype, device=self.device)
+            self._bb_est = est
+        return est
+
+    def T1(self, eta, G_eta, boson):
+        """stochastic_estimator.py:1393-1445; boson [1, 2, Lx, Ly, Ltau]; returns [Ly, Lx]."""
+        return self._bb(eta).T("T1", eta, G_eta, boson)
+
+    def T21(self, eta, G_eta, boson):
+        """stochastic_estimator.py:1447-1502."""
+        return self._bb(eta).T("T21", eta, G_eta, boson)
+
+    def T2(self, eta, G_eta, boson):
+        """stochastic_estimator.py:1504-1559."""
+        return self._bb(eta).T("T2", eta, G_eta, boson)
+
+    def T4(self, eta, G_eta, boson):
+        """stochastic_estimator.py:1561-1610."""
+        return self._bb(eta).T("T4", eta, G_eta, boson)
+
+    def Tvv(self, eta, G_eta, boson):
+        """(Tvv, Tv) (stochastic_estimator.py:1612-1654)."""
+        return self._bb(eta).Tvv(eta, G_eta, boson)
+
+    def get_bond_bond_per_b2(self, boson):
+        """BB0_r, BB_r, B_r, BB_k of one chain from self.eta / self.G_eta (stochastic_estimator.py:2594-2641)."""
+        runners = None
+        if self.cuda_graph_se and self.T1_graph_runner.graph is not None:
+            runners = {"T1": self.T1_graph_runner, "T21": self.T21_graph_runner, "T2": self.T2_graph_runner,
+                       "T4": self.T4_graph_runner, "Tvv": self.Tvv_graph_runner}
+        return self._bb(self.eta).bond_bond(self.eta, self.G_eta, boson, runners)
+
+    @torch.inference_mode()
+    def get_fermion_obsr_compile(self, bosons, eta):
+        """Per chain: G_eta solve, then the bond-bond observables, stacked to [bs, Ly, Lx]
+        (stochastic_estimator.py:2492-2522)."""
+        obsrs = []
+        for b in range(bosons.shape[0]):
+            boson = bosons[b].unsqueeze(0)
+            self.set_eta_G_eta(boson, eta)
+            obsrs.append(self.get_bond_bond_per_b2(boson))
+        return {k: torch.stack([o[k] for o in obsrs], dim=0) for k in obsrs[0]}
+
+    def bond_corr(self, BB_r_mean, B_r_mean):
+        """stochastic_estimator.py:2643-2651."""
+        return observables.bond_corr(BB_r_mean, B_r_mean)
+
+    reorder_fft_grid2 = staticmethod(observables.reorder_fft_grid2)
+    reorder_fft_grid2_inverse = staticmethod(observables.reorder_fft_grid2_inverse)
