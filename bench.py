@@ -156,6 +156,22 @@ def extras(torch, K, args, dev):
                             "mdagm_GBps": (nrv * 16 * N + 16 * N) / t_op / 1e9, "mdagm_ms": t_op * 1e3}
     log(f"[bench] SE L=48 128 RHS: {t_se:.3f} s per {it}-iteration solve, M^dag M {out['se_L48_128rhs']['mdagm_GBps']:.0f} GB/s")
     del eta, b, x, hop
+    # -- bond-bond estimators of one chain (get_bond_bond_per_b2: T1, T21, T2, T4 over all
+    #    C(40, 2) pairs, Tvv), L=36, Ltau=360, Nrv=40 (the reference's Nrv at this size)
+    from qed_fermion_amd.observables import BondBondEstimator
+    L, Lt, nrv = 36, 360, 40
+    N = L * L * Lt
+    z2 = lambda: (torch.randint(0, 2, (nrv, N), generator=g) * 2 - 1).to(torch.float32)
+    eta_b = (torch.complex(z2(), z2()) / math.sqrt(2)).to(dev)
+    G_b = torch.randn(nrv, N, generator=g, dtype=torch.complex64).to(dev)
+    bos = ((torch.rand(1, 2, L, L, Lt, generator=g, dtype=torch.float64) * 2 - 1) * math.pi).to(dev, torch.float32)
+    est = BondBondEstimator(L, Lt, nrv, device=dev)
+    t_bb = timed(torch, lambda: est.bond_bond(eta_b, G_b, bos), reps=3)
+    out["bond_bond_L36_nrv40"] = {"s_per_chain": t_bb, "pairs": nrv * (nrv - 1) // 2, "pair_chunk": est.chunk,
+                                  "estimators": ["T1", "T21", "T2", "T4", "Tvv"],
+                                  "note": "get_bond_bond_per_b2 after the G_eta solve; G_eta synthetic (the cost does not depend on it)"}
+    log(f"[bench] bond-bond estimators L=36 Nrv=40: {t_bb * 1e3:.1f} ms per chain")
+    del eta_b, G_b, est
     # -- batching: ragged set, L in {12, 18, 24, 30} x 2, Ltau = 10 L, one launch per kernel
     Ls = [12, 18, 24, 30, 12, 18, 24, 30]
     plan = K.RaggedPlan(Ls, [10 * v for v in Ls], device=dev)

@@ -39,7 +39,7 @@ class HmcSampler(object):
 
     def __init__(self, Lx=6, Ltau=10, J=0.5, Nstep=3000, config=None, *, bs=None, dtau=None, K=None, compact=None,
                  precon_on=None, max_iter=None, cg_rtol=1e-9, cuda_graph=None, precision="fp32", ru_period=50,
-                 device="cuda", seed=None, ckpt_root=None):
+                 device="cuda", seed=None, ckpt_root=None, max_tau_block_idx=None):
         self.Lx = Lx
         self.Ly = Lx
         self.Ltau = Ltau
@@ -75,10 +75,15 @@ class HmcSampler(object):
         self.delta_t_tensor = torch.full((self.bs,), self.delta_t, device=self.device, dtype=self.dtype)
         self.lower_limit = 0.25
         self.upper_limit = 0.5
-        self.threshold_queue = [collections.deque(maxlen=5) for _ in range(self.bs)]
-        self.max_tau_block_idx = 1
+        # tau-block updates (hmc_sampler_batch.py:158-162): max_tau_block_idx blocks of Ltau // blocks
+        # slices, one Metropolis step each, only when Ltau >= 10 Lx
+        nblk = max_tau_block_idx if max_tau_block_idx is not None else _env("max_tau_block_idx", 1, int)
+        self.max_tau_block_idx = nblk if self.Ltau // self.Lx // 10 >= 1 else 1
         self.tau_block_idx = 0
-        self.tau_block_size = self.Ltau
+        self.tau_block_size = self.Ltau // self.max_tau_block_idx
+        self.threshold_queue = [collections.deque(maxlen=5 * self.max_tau_block_idx) for _ in range(self.bs)]
+        # CG iteration gears (hmc_sampler_batch.py:305-309): one gear unless set otherwise
+        self._MAX_ITERS_TO_CAPTURE = [self.max_iter] if self.dtau <= 0.1 and self.precon_on else [400]
 
         self.precon = None
         self.precon_csr = None
@@ -327,14 +332,28 @@ class HmcSampler(object):
         return x, eng.H0.clone(), eng.H1.clone(), eng.iters.mean(dim=0), eng.err.mean(dim=0)
 
     def metropolis_update(self):
-        """Propose, accept with rand < exp(H_old - H_new), record thresholds (hmc_sampler_batch.py:2886-2930)."""
-        tau_mask = torch.ones((1, 1, 1, 1, self.Ltau), device=self.device, dtype=self.dtype)
-        boson_new, H_old, H_new, cg_converge_iter, cg_r_err = self.leapfrog_proposer5(self.boson, tau_mask)
-        accp = torch.rand(self.bs, device=self.device, dtype=torch.float64) < torch.exp(H_old - H_new)
-        self.boson[accp] = boson_new[accp]
-        thresholds = torch.minimum(torch.exp(H_old - H_new), torch.ones_like(H_old))
-        for b in range(self.bs):
-            self.threshold_queue[b].append(thresholds[b].item())
+        """One Metropolis step per tau block: propose with the block's tau mask, accept with
+        rand < exp(H_old - H_new), record thresholds (hmc_sampler_batch.py:2886-2930).  With
+        several CG gears, the first applies after gear0_steps sweeps (2896-2899)."""
+        gears = self._MAX_ITERS_TO_CAPTURE
+        gear0_steps = _env("gear0_steps", 1000, int)
+        want = gears[0] if len(gears) == 1 or self.cur_step > gear0_steps else gears[1]
+        if want != self.max_iter:
+            self.max_iter = want
+            self._engine = None
+        accp = cg_converge_iter = cg_r_err = None
+        while self.tau_block_idx < self.max_tau_block_idx:
+            t0 = self.tau_block_idx * self.tau_block_size
+            tau_mask = torch.zeros((1, 1, 1, 1, self.Ltau), device=self.device, dtype=self.dtype)
+            tau_mask[..., t0:t0 + self.tau_block_size] = 1.0
+            boson_new, H_old, H_new, cg_converge_iter, cg_r_err = self.leapfrog_proposer5(self.boson, tau_mask)
+            accp = torch.rand(self.bs, device=self.device, dtype=torch.float64) < torch.exp(H_old - H_new)
+            self.boson[accp] = boson_new[accp]
+            thresholds = torch.minimum(torch.exp(H_old - H_new), torch.ones_like(H_old))
+            for b in range(self.bs):
+                self.threshold_queue[b].append(thresholds[b].item())
+            self.tau_block_idx += 1
+        self.tau_block_idx = 0
         return self.boson, accp, cg_converge_iter, cg_r_err
 
     def adjust_delta_t(self):

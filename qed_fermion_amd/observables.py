@@ -47,7 +47,7 @@ class BondBondEstimator:
     """Workspaces and launches for one (L, Ltau, Nrv, dtype, device).
 
     pair_chunk: pairs per chunk (default: as many as fit ``chunk_bytes`` per
-    field buffer; four such buffers are held).  nwg: work-groups of the
+    field buffer; two such buffers are held, and the transforms of a chunk take two more).  nwg: work-groups of the
     k-space accumulation (its FP64 partials are [nwg, L*L])."""
 
     def __init__(self, L, Ltau, nrv, dtype=torch.complex64, device="cuda", pair_chunk=None, chunk_bytes=256 << 20, nwg=512):
@@ -72,8 +72,6 @@ class BondBondEstimator:
         self.pairs = torch.combinations(torch.arange(nrv), r=2).to(torch.int32).reshape(-1).contiguous().to(dev)
         self.A = torch.empty(self.chunk, self.N, dtype=dtype, device=dev)
         self.B = torch.empty_like(self.A)
-        self.Ah = torch.empty_like(self.A)
-        self.Bh = torch.empty_like(self.A)
         self.part = torch.empty(2 * self.nwg, self.V, dtype=torch.complex128, device=dev)
         self.S = torch.empty(2, self.V, dtype=torch.complex128, device=dev)
         self.dpart = torch.empty(self.nblk, 3, dtype=torch.complex128, device=dev)
@@ -94,9 +92,11 @@ class BondBondEstimator:
         phix = boson.reshape(-1, 2, L, L, Lt)[0, 0].permute(2, 1, 0).to(_REAL[self.dtype]).contiguous()   # [tau, y, x]
         return eta.contiguous(), G_eta.contiguous(), phix
 
-    def _fft(self, src, dst, n):
+    def _fft(self, src, n):
+        """Forward 2-D transforms of the first n fields (torch's rocFFT plan; writing into a given
+        out= buffer would add a full copy of the result)."""
         L = self.L
-        torch.fft.fft2(src[:n].view(n * self.Ltau, L, L), out=dst[:n].view(n * self.Ltau, L, L))
+        return torch.fft.fft2(src[:n].view(n * self.Ltau, L, L))
 
     def _kspace_sum(self, obs, eta, G, phix, count):
         """S[0] = sum_k-products over all pairs (or noise vectors for OBS_V), S[1] = sum of Ahat (OBS_V)."""
@@ -108,13 +108,9 @@ class BondBondEstimator:
             n = min(self.chunk, count - p0)
             call("qf_obs_fields", prec, obs, ptr(eta), ptr(G), ptr(phix), ptr(self.pairs), p0, n, self.nrv, self.L, self.Ltau,
                  ptr(self.A), ptr(self.B), st)
-            self._fft(self.A, self.Ah, n)
-            if lin:
-                bh = self.Ah
-            else:
-                self._fft(self.B, self.Bh, n)
-                bh = self.Bh
-            call("qf_obs_kspace", prec, ptr(self.Ah), ptr(bh), n * self.Ltau, self.L, lin, 0 if first else 1, ptr(self.part),
+            ah = self._fft(self.A, n)
+            bh = ah if lin else self._fft(self.B, n)
+            call("qf_obs_kspace", prec, ptr(ah), ptr(bh), n * self.Ltau, self.L, lin, 0 if first else 1, ptr(self.part),
                  self.nwg, st)
             first = False
         call("qf_obs_reduce", ptr(self.part), self.nwg, self.V, ptr(self.S[0]), st)

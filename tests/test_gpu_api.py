@@ -157,6 +157,29 @@ def test_metropolis_update_runs(dev, cuda_graph):
     assert all(0.0 <= v <= 1.0 for q in hmc.threshold_queue for v in q)
 
 
+@pytest.mark.parametrize("cuda_graph", [False, True])
+def test_tau_block_updates(dev, cuda_graph):
+    """Two tau blocks (hmc_sampler_batch.py:2901-2925): a proposal with block 0's mask leaves
+    block 1 of the field untouched, and metropolis_update runs one accept step per block."""
+    L, Lt, bs = 4, 80, 2                      # Ltau >= 10 Lx, so blocks are allowed
+    hmc = _sampler(L, Lt, bs, "fp64", max_iter=200, cuda_graph=cuda_graph, max_tau_block_idx=2)
+    assert hmc.max_tau_block_idx == 2 and hmc.tau_block_size == 40
+    assert hmc.threshold_queue[0].maxlen == 10
+    hmc.delta_t_tensor.fill_(0.02)
+    mask = torch.zeros((1, 1, 1, 1, Lt), device=dev, dtype=hmc.dtype)
+    mask[..., :40] = 1.0
+    x, H0, H1, _, _ = hmc.leapfrog_proposer5(hmc.boson, mask)
+    torch.cuda.synchronize()
+    assert torch.equal(x[..., 40:], hmc.boson[..., 40:])
+    assert not torch.equal(x[..., :40], hmc.boson[..., :40])
+    assert torch.isfinite(H0).all() and torch.isfinite(H1).all()
+    boson, accp, iters, r_err = hmc.metropolis_update()
+    torch.cuda.synchronize()
+    assert all(len(q) == 2 for q in hmc.threshold_queue) and hmc.tau_block_idx == 0
+    small = _sampler(4, 30, bs, "fp64", max_tau_block_idx=2)   # Ltau < 10 Lx: one block
+    assert small.max_tau_block_idx == 1 and small.tau_block_size == 30
+
+
 def test_stochastic_estimator_solve(dev):
     from qed_fermion_amd.stochastic_estimator import StochaticEstimator
     L, Lt = 4, 30
