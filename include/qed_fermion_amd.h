@@ -203,6 +203,16 @@ int qf_obs_reduce(const void* part, int nw, int n, void* out, void* stream);
  * 1: G(r) e(r) (T21, T2), 2: G(r) e(r+2x) conj(U(r) U(r+x)) (T4); e = conj(eta),
  * U = exp(i phix). */
 int qf_obs_diag(int prec, const void* eta, const void* G, const void* phix, int nrv, int L, int Ltau, void* part, int nblk, void* stream);
+/* The whole pair sum of one estimator in one launch, no FFT library: each
+ * work-group builds a and b of one (tau slice, pair) in LDS, transforms both
+ * with its own mixed-radix FFT and adds Ahat(k) Bhat(-k) in FP64 registers.
+ * part: [nwg][L*L] double2 (QF_OBS_V: [2*nwg][L*L], the second half the sums
+ * of Ahat), reduced with qf_obs_reduce.  count: pairs (QF_OBS_V: noise
+ * vectors, <= nrv).  Returns QF_ERR_UNSUPPORTED where the slices do not fit
+ * in LDS (complex128 with L > 58, any L > 78). */
+int qf_obs_fused_grid(int prec, int L);   /* work-groups that fill the device; 0 = unsupported */
+int qf_obs_fused(int prec, int obs, const void* eta, const void* G, const void* phix, const int* pairs, int count, int nrv, int L, int Ltau,
+                 void* part, int nwg, void* stream);
 
 #ifdef __cplusplus
 }

@@ -82,6 +82,8 @@ SIGNATURES = {
     "qf_obs_kspace": (_I, [_I, _P, _P, _LL, _I, _I, _I, _P, _I, _P]),
     "qf_obs_reduce": (_I, [_P, _I, _I, _P, _P]),
     "qf_obs_diag": (_I, [_I, _P, _P, _P, _I, _I, _I, _P, _I, _P]),
+    "qf_obs_fused_grid": (_I, [_I, _I]),
+    "qf_obs_fused": (_I, [_I, _I, _P, _P, _P, _P, _I, _I, _I, _I, _P, _I, _P]),
 }
 
 _lib = None

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/qed_fermion_amd.h"
@@ -545,6 +546,28 @@ const Band kBand[7] = {{kEdge0, 10, kTail0, 9},  {kEdge1, 12, kTail1, 8}, {kEdge
 }  // namespace
 
 // =================================================================== C ABI
+namespace {
+
+// Radices of a length-L transform for the fused estimator kernel: 4s, then 2,
+// 3, 5, 7 and any larger prime factors; false if more than kObsMaxFac.
+bool obs_factor(int L, unsigned long long& facs, int& nfac) {
+    facs = 0;
+    nfac = 0;
+    int n = L;
+    auto push = [&](int p) {
+        if (nfac >= kObsMaxFac) return false;
+        facs |= (unsigned long long)p << (8 * nfac++);
+        return true;
+    };
+    while (n % 4 == 0) { if (!push(4)) return false; n /= 4; }
+    for (int p = 2; p <= n; ++p)
+        while (n % p == 0) { if (!push(p)) return false; n /= p; }
+    return true;
+}
+
+
+}  // namespace
+
 extern "C" {
 
 int qf_version(void) { return QF_ABI_VERSION; }
@@ -1145,5 +1168,91 @@ int qf_obs_diag(int prec, const void* eta, const void* G, const void* phix, int 
     return QF_OK;
 }
 
+int qf_obs_fused_grid(int prec, int L) {
+    if (L < 2 || L > 78) return 0;
+    unsigned long long f;
+    int n;
+    if (!obs_factor(L, f, n)) return 0;
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1) return 0;
+    const int V = L * L;
+    // resident work-groups per CU of the least-occupying estimator kernel at this L
+    auto occ = [&](auto tag) -> int {
+        using R = decltype(tag);
+        const size_t lds = obs_fused_lds<R>(L);
+        if (lds > 160 * 1024) return 0;
+        int best = 1 << 30;
+        auto one = [&](auto kernel) {
+            int b = 0;
+            if (set_lds(kernel, lds) == QF_OK && hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, 256, lds) == hipSuccess)
+                best = std::min(best, b);
+            else
+                best = 0;
+        };
+        auto all = [&](auto k4, auto k8, auto k16, auto k24) {
+            if (V <= 1024) one(k4); else if (V <= 2048) one(k8); else if (V <= 4096) one(k16); else one(k24);
+        };
+        all(obs_fused_kernel<R, OBS_T1, 4>, obs_fused_kernel<R, OBS_T1, 8>, obs_fused_kernel<R, OBS_T1, 16>, obs_fused_kernel<R, OBS_T1, 24>);
+        all(obs_fused_kernel<R, OBS_T21, 4>, obs_fused_kernel<R, OBS_T21, 8>, obs_fused_kernel<R, OBS_T21, 16>,
+            obs_fused_kernel<R, OBS_T21, 24>);
+        all(obs_fused_kernel<R, OBS_T2, 4>, obs_fused_kernel<R, OBS_T2, 8>, obs_fused_kernel<R, OBS_T2, 16>, obs_fused_kernel<R, OBS_T2, 24>);
+        all(obs_fused_kernel<R, OBS_T4, 4>, obs_fused_kernel<R, OBS_T4, 8>, obs_fused_kernel<R, OBS_T4, 16>, obs_fused_kernel<R, OBS_T4, 24>);
+        all(obs_fused_kernel<R, OBS_V, 4>, obs_fused_kernel<R, OBS_V, 8>, obs_fused_kernel<R, OBS_V, 16>, obs_fused_kernel<R, OBS_V, 24>);
+        return best == (1 << 30) ? 0 : best;
+    };
+    const int b = prec == QF_PREC_F32 ? occ(float{}) : prec == QF_PREC_F64 ? occ(double{}) : 0;
+    return std::min(b * ncu, 65535);
+}
+
+int qf_obs_fused(int prec, int obs, const void* eta, const void* G, const void* phix, const int* pairs, int count, int nrv, int L, int Ltau,
+                 void* part, int nwg, void* stream) {
+    if (!eta || !G || !phix || !part) return fail(QF_ERR_INVALID, "null pointer");
+    if (obs < QF_OBS_T1 || obs > QF_OBS_V) return fail(QF_ERR_INVALID, "estimator %d", obs);
+    if (obs != QF_OBS_V && !pairs) return fail(QF_ERR_INVALID, "pair estimators need pairs");
+    if (L < 2 || Ltau < 1 || nrv < 1 || count < 1 || nwg < 1 || nwg > 65535) return fail(QF_ERR_INVALID, "bad sizes");
+    if (obs == QF_OBS_V && count > nrv) return fail(QF_ERR_INVALID, "%d noise vectors beyond nrv %d", count, nrv);
+    if (L > 78) return fail(QF_ERR_UNSUPPORTED, "fused estimator kernel: L = %d > 78", L);
+    unsigned long long facs;
+    int nfac;
+    if (!obs_factor(L, facs, nfac)) return fail(QF_ERR_UNSUPPORTED, "fused estimator kernel: L = %d has too many factors", L);
+    hipStream_t s = (hipStream_t)stream;
+    const int V = L * L;
+    auto run = [&](auto tag) -> int {
+        using R = decltype(tag);
+        const size_t lds = obs_fused_lds<R>(L);
+        if (lds > 160 * 1024) return fail(QF_ERR_UNSUPPORTED, "fused estimator kernel: %zu bytes of LDS at L = %d", lds, L);
+        ObsFusedArgs<R> a{};
+        a.eta = (const cplx<R>*)eta; a.G = (const cplx<R>*)G; a.phix = (const R*)phix; a.pairs = pairs;
+        a.count = count; a.nrv = nrv; a.L = L; a.Lt = Ltau; a.N = (long long)V * Ltau; a.nfac = nfac; a.facs = facs;
+        a.part = (double2*)part;
+        auto go = [&](auto kernel) -> int {
+            int rc = set_lds(kernel, lds);
+            if (rc) return rc;
+            hipLaunchKernelGGL(kernel, dim3(nwg), dim3(256), lds, s, a);
+            QF_LAUNCHED();
+            return QF_OK;
+        };
+        auto by_k = [&](auto obs_tag) -> int {
+            constexpr int O = decltype(obs_tag)::value;
+            if (V <= 1024) return go(obs_fused_kernel<R, O, 4>);
+            if (V <= 2048) return go(obs_fused_kernel<R, O, 8>);
+            if (V <= 4096) return go(obs_fused_kernel<R, O, 16>);
+            return go(obs_fused_kernel<R, O, 24>);
+        };
+        switch (obs) {
+            case QF_OBS_T1: return by_k(std::integral_constant<int, OBS_T1>{});
+            case QF_OBS_T21: return by_k(std::integral_constant<int, OBS_T21>{});
+            case QF_OBS_T2: return by_k(std::integral_constant<int, OBS_T2>{});
+            case QF_OBS_T4: return by_k(std::integral_constant<int, OBS_T4>{});
+            default: return by_k(std::integral_constant<int, OBS_V>{});
+        }
+    };
+    if (prec == QF_PREC_F32) return run(float{});
+    if (prec == QF_PREC_F64) return run(double{});
+    return fail(QF_ERR_INVALID, "prec %d", prec);
+}
+
 }  // extern "C"
+
 

@@ -45,13 +45,48 @@ template <> __device__ __forceinline__ double2 expi<double>(double p) {
     return make_double2(c, s);
 }
 
-// Fields of one estimator for each (pair, site) of the launch.  With
-// e = eta*, x+ the site r + x, U = e^{i phix(r)}, Ub = conj(U):
-//   T1  a = e_s(r) G_s'(r+) U                          b = G_s(r+) e_s'(r) U            (1419-1425)
-//   T21 a = (e_s(r+) G_s'(r) + e_s'(r+) G_s(r)) Ub/2    b = (G_s(r+) e_s'(r) + G_s'(r+) e_s(r)) U/2   (1473-1487)
-//   T2  a = (e_s(r) G_s'(r+) + e_s'(r) G_s(r+)) U/2     b = (G_s(r) e_s'(r+) + G_s'(r) e_s(r+)) Ub/2  (1530-1544)
-//   T4  a = e_s(r+) G_s'(r) Ub                          b = G_s(r) e_s'(r+) Ub           (1587-1593)
-//   V   a = G_s(r+) e_s(r) U + G_s(r) e_s(r+) Ub  (one noise vector s)                   (1631-1636)
+// Fields of one estimator at one site of noise vectors s, t (t = s for OBS_V).
+// With e = eta*, x+ the site r + x, U = e^{i phix(r)}, Ub = conj(U):
+//   T1  a = e_s(r) G_t(r+) U                         b = G_s(r+) e_t(r) U            (1419-1425)
+//   T21 a = (e_s(r+) G_t(r) + e_t(r+) G_s(r)) Ub/2    b = (G_s(r+) e_t(r) + G_t(r+) e_s(r)) U/2   (1473-1487)
+//   T2  a = (e_s(r) G_t(r+) + e_t(r) G_s(r+)) U/2     b = (G_s(r) e_t(r+) + G_t(r) e_s(r+)) Ub/2  (1530-1544)
+//   T4  a = e_s(r+) G_t(r) Ub                         b = G_s(r) e_t(r+) Ub           (1587-1593)
+//   V   a = G_s(r+) e_s(r) U + G_s(r) e_s(r+) Ub                                     (1631-1636)
+// site indexes a vector (tau * V + y * L + x), xp is its x-neighbour.
+template <typename R, int OBS>
+__device__ __forceinline__ void obs_site(const cplx<R>* eta, const cplx<R>* G, const R* phix, long long N, int s, int t, long long site,
+                                         long long xp, cplx<R>& va, cplx<R>& vb) {
+    const cplx<R>* es = eta + (long long)s * N;
+    const cplx<R>* gs = G + (long long)s * N;
+    const cplx<R> U = expi<R>(phix[site]);
+    const cplx<R> Ub = cconj<R>(U);
+    const cplx<R> e0 = cconj<R>(es[site]), e1 = cconj<R>(es[xp]);
+    const cplx<R> g0 = gs[site], g1 = gs[xp];
+    if constexpr (OBS == OBS_V) {
+        va = cadd<R>(cmul<R>(cmul<R>(g1, e0), U), cmul<R>(cmul<R>(g0, e1), Ub));
+        vb = va;
+    } else {
+        const cplx<R>* et = eta + (long long)t * N;
+        const cplx<R>* gt = G + (long long)t * N;
+        const cplx<R> f0 = cconj<R>(et[site]), f1 = cconj<R>(et[xp]);
+        const cplx<R> h0 = gt[site], h1 = gt[xp];
+        if constexpr (OBS == OBS_T1) {
+            va = cmul<R>(cmul<R>(e0, h1), U);
+            vb = cmul<R>(cmul<R>(g1, f0), U);
+        } else if constexpr (OBS == OBS_T21) {
+            va = cscale<R>((R)0.5, cmul<R>(cadd<R>(cmul<R>(e1, h0), cmul<R>(f1, g0)), Ub));
+            vb = cscale<R>((R)0.5, cmul<R>(cadd<R>(cmul<R>(g1, f0), cmul<R>(h1, e0)), U));
+        } else if constexpr (OBS == OBS_T2) {
+            va = cscale<R>((R)0.5, cmul<R>(cadd<R>(cmul<R>(e0, h1), cmul<R>(f0, g1)), U));
+            vb = cscale<R>((R)0.5, cmul<R>(cadd<R>(cmul<R>(g0, f1), cmul<R>(h0, e1)), Ub));
+        } else {
+            va = cmul<R>(cmul<R>(e1, h0), Ub);
+            vb = cmul<R>(cmul<R>(g0, f1), Ub);
+        }
+    }
+}
+
+// Fields of every (pair, site) of the launch into A, B (rocFFT path).
 template <typename R, int OBS>
 __global__ void __launch_bounds__(256) obs_fields_kernel(ObsArgs<R> a) {
     const long long total = (long long)a.np * a.N;
@@ -74,36 +109,10 @@ __global__ void __launch_bounds__(256) obs_fields_kernel(ObsArgs<R> a) {
             if constexpr (OBS != OBS_V) a.B[i] = mk<R>(0, 0);
             continue;
         }
-        const cplx<R>* es = a.eta + (long long)s * a.N;
-        const cplx<R>* gs = a.G + (long long)s * a.N;
-        const cplx<R> U = expi<R>(a.phix[site]);
-        const cplx<R> Ub = cconj<R>(U);
-        const cplx<R> e0 = cconj<R>(es[site]), e1 = cconj<R>(es[xp]);
-        const cplx<R> g0 = gs[site], g1 = gs[xp];
-        if constexpr (OBS == OBS_V) {
-            a.A[i] = cadd<R>(cmul<R>(cmul<R>(g1, e0), U), cmul<R>(cmul<R>(g0, e1), Ub));
-        } else {
-            const cplx<R>* et = a.eta + (long long)t * a.N;
-            const cplx<R>* gt = a.G + (long long)t * a.N;
-            const cplx<R> f0 = cconj<R>(et[site]), f1 = cconj<R>(et[xp]);
-            const cplx<R> h0 = gt[site], h1 = gt[xp];
-            cplx<R> va, vb;
-            if constexpr (OBS == OBS_T1) {
-                va = cmul<R>(cmul<R>(e0, h1), U);
-                vb = cmul<R>(cmul<R>(g1, f0), U);
-            } else if constexpr (OBS == OBS_T21) {
-                va = cscale<R>((R)0.5, cmul<R>(cadd<R>(cmul<R>(e1, h0), cmul<R>(f1, g0)), Ub));
-                vb = cscale<R>((R)0.5, cmul<R>(cadd<R>(cmul<R>(g1, f0), cmul<R>(h1, e0)), U));
-            } else if constexpr (OBS == OBS_T2) {
-                va = cscale<R>((R)0.5, cmul<R>(cadd<R>(cmul<R>(e0, h1), cmul<R>(f0, g1)), U));
-                vb = cscale<R>((R)0.5, cmul<R>(cadd<R>(cmul<R>(g0, f1), cmul<R>(h0, e1)), Ub));
-            } else {
-                va = cmul<R>(cmul<R>(e1, h0), Ub);
-                vb = cmul<R>(cmul<R>(g0, f1), Ub);
-            }
-            a.A[i] = va;
-            a.B[i] = vb;
-        }
+        cplx<R> va, vb;
+        obs_site<R, OBS>(a.eta, a.G, a.phix, a.N, s, t, site, xp, va, vb);
+        a.A[i] = va;
+        if constexpr (OBS != OBS_V) a.B[i] = vb;
     }
 }
 
@@ -197,5 +206,198 @@ __global__ void __launch_bounds__(256) obs_diag_kernel(const cplx<R>* eta, const
     }
     if (t < 3) part[(long long)blockIdx.x * 3 + t] = make_double2(red[2 * t][0], red[2 * t + 1][0]);
 }
+
+// ---------------------------------------------------------------- fused path
+// One work-group takes items (tau slice, pair) in turn: it builds a and b of
+// the slice in LDS, transforms both in place with a mixed-radix Stockham
+// autosort FFT (the per-stage formulation of Govindaraju et al., "High
+// performance discrete Fourier transforms on graphics processors", SC 2008:
+// stage of radix P with Ns = product of the earlier radices reads
+// x_r = src[j + r L/P] e^{-2 pi i r (j mod Ns)/(Ns P)}, takes their P-point DFT
+// and writes it to dst[(j - j mod Ns) P + j mod Ns + q Ns]), rows then
+// columns, and adds Ahat(k) Bhat(-k) to FP64 registers.  Only the inputs (from
+// L2) and one partial per work-group touch global memory.
+
+constexpr int kObsMaxFac = 8;   // radices of one transform (L <= 78 needs at most 4)
+
+template <typename R> struct ObsFusedArgs {
+    const cplx<R>* eta;
+    const cplx<R>* G;
+    const R* phix;
+    const int* pairs;
+    int count;                  // pairs (OBS_V: noise vectors)
+    int nrv, L, Lt;
+    long long N;
+    int nfac;
+    unsigned long long facs;    // radices in stage order, 8 bits each (stage f: bits 8f..8f+7), product L
+    double2* part;              // [gridDim.x][V] (+ [gridDim.x][V] sums of Ahat for OBS_V)
+};
+
+template <typename R>
+__device__ __forceinline__ cplx<R> tw_mul(cplx<R> u, const cplx<R>* tw, int e) {
+    return e ? cmul<R>(u, tw[e]) : u;
+}
+
+// One Stockham stage of compile-time radix P over L lines of length L
+// (element i of line l at l * ls + i * es); tw[m] = e^{-2 pi i m / L}.
+template <typename R, int P>
+__device__ __forceinline__ void stockham_stage(const cplx<R>* src, cplx<R>* dst, const cplx<R>* tw, int L, int Ns, int ls, int es) {
+    const int nb = L / P, total = L * nb, tstep = L / (Ns * P), wstep = L / P;
+    for (int b = threadIdx.x; b < total; b += blockDim.x) {
+        const int line = b / nb, j = b - line * nb, k = j % Ns;
+        const int sb = line * ls;
+        cplx<R> v[P];
+#pragma unroll
+        for (int r = 0; r < P; ++r) v[r] = tw_mul<R>(src[sb + (j + r * nb) * es], tw, (r * k * tstep) % L);
+        const int d0 = sb + ((j - k) * P + k) * es, dq = Ns * es;
+        if constexpr (P == 2) {
+            dst[d0] = cadd<R>(v[0], v[1]);
+            dst[d0 + dq] = csub<R>(v[0], v[1]);
+        } else if constexpr (P == 3) {
+            const R h = (R)0.86602540378443864676;    // sin(2 pi / 3)
+            const cplx<R> sm = cadd<R>(v[1], v[2]), df = csub<R>(v[1], v[2]);
+            const cplx<R> m = mk<R>(v[0].x - (R)0.5 * sm.x, v[0].y - (R)0.5 * sm.y);
+            dst[d0] = cadd<R>(v[0], sm);
+            dst[d0 + dq] = mk<R>(m.x + h * df.y, m.y - h * df.x);           // m - i h df
+            dst[d0 + 2 * dq] = mk<R>(m.x - h * df.y, m.y + h * df.x);       // m + i h df
+        } else if constexpr (P == 4) {
+            const cplx<R> a = cadd<R>(v[0], v[2]), bb = csub<R>(v[0], v[2]);
+            const cplx<R> c = cadd<R>(v[1], v[3]), d = csub<R>(v[1], v[3]);
+            dst[d0] = cadd<R>(a, c);
+            dst[d0 + dq] = mk<R>(bb.x + d.y, bb.y - d.x);                    // b - i d
+            dst[d0 + 2 * dq] = csub<R>(a, c);
+            dst[d0 + 3 * dq] = mk<R>(bb.x - d.y, bb.y + d.x);                // b + i d
+        } else {
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                cplx<R> y = v[0];
+#pragma unroll
+                for (int r = 1; r < P; ++r) y = cadd<R>(y, cmul<R>(v[r], tw[((q * r) % P) * wstep]));
+                dst[d0 + q * dq] = y;
+            }
+        }
+    }
+}
+
+// The same for any radix P (larger primes): inputs are re-read from LDS per output.
+template <typename R>
+__device__ __forceinline__ void stockham_stage_any(const cplx<R>* src, cplx<R>* dst, const cplx<R>* tw, int L, int P, int Ns, int ls,
+                                                   int es) {
+    const int nb = L / P, total = L * nb, tstep = L / (Ns * P), wstep = L / P;
+    for (int b = threadIdx.x; b < total; b += blockDim.x) {
+        const int line = b / nb, j = b - line * nb, k = j % Ns;
+        const int sb = line * ls;
+        const int d0 = sb + ((j - k) * P + k) * es;
+        for (int q = 0; q < P; ++q) {
+            cplx<R> y = mk<R>(0, 0);
+            for (int r = 0; r < P; ++r) {
+                const cplx<R> u = tw_mul<R>(src[sb + (j + r * nb) * es], tw, (r * k * tstep) % L);
+                y = cadd<R>(y, tw_mul<R>(u, tw, ((q * r) % P) * wstep));
+            }
+            dst[d0 + q * Ns * es] = y;
+        }
+    }
+}
+
+// Forward 2-D transform of X ([L][L], in LDS) in place, T as the ping-pong
+// buffer: rows, then columns; 2 * nfac stages leave the result in X.
+template <typename R>
+__device__ void stockham_2d(cplx<R>* X, cplx<R>* T, const cplx<R>* tw, int L, unsigned long long facs, int nfac) {
+    cplx<R>* src = X;
+    cplx<R>* dst = T;
+    for (int pass = 0; pass < 2; ++pass) {
+        const int ls = pass ? 1 : L, es = pass ? L : 1;
+        int Ns = 1;
+        for (int f = 0; f < nfac; ++f) {
+            const int P = (int)((facs >> (8 * f)) & 0xFF);
+            switch (P) {
+                case 2: stockham_stage<R, 2>(src, dst, tw, L, Ns, ls, es); break;
+                case 3: stockham_stage<R, 3>(src, dst, tw, L, Ns, ls, es); break;
+                case 4: stockham_stage<R, 4>(src, dst, tw, L, Ns, ls, es); break;
+                case 5: stockham_stage<R, 5>(src, dst, tw, L, Ns, ls, es); break;
+                case 7: stockham_stage<R, 7>(src, dst, tw, L, Ns, ls, es); break;
+                default: stockham_stage_any<R>(src, dst, tw, L, P, Ns, ls, es); break;
+            }
+            __syncthreads();
+            Ns *= P;
+            cplx<R>* t = src;
+            src = dst;
+            dst = t;
+        }
+    }
+}
+
+// part[w][k] = sum over the work-group's items of Ahat(k) Bhat(-k); for OBS_V
+// (Bhat = Ahat) also part[gridDim.x + w][k] = sum Ahat(k).  MAXK >= L*L / 256.
+template <typename R, int OBS, int MAXK>
+__global__ void __launch_bounds__(256) obs_fused_kernel(ObsFusedArgs<R> a) {
+    extern __shared__ __align__(16) unsigned char obs_lds[];
+    const int L = a.L, V = L * L;
+    cplx<R>* tw = reinterpret_cast<cplx<R>*>(obs_lds);
+    cplx<R>* A = tw + L;
+    cplx<R>* B = A + V;
+    cplx<R>* T = B + V;
+    for (int j = threadIdx.x; j < L; j += blockDim.x) {
+        double sn, cs;
+        sincos(-6.283185307179586476925 * j / L, &sn, &cs);
+        tw[j] = mk<R>((R)cs, (R)sn);
+    }
+    double pr[MAXK], pi[MAXK], lr[MAXK], li[MAXK];
+#pragma unroll
+    for (int m = 0; m < MAXK; ++m) pr[m] = pi[m] = lr[m] = li[m] = 0.0;
+    __syncthreads();
+    const long long items = (long long)a.count * a.Lt;
+    for (long long it = blockIdx.x; it < items; it += gridDim.x) {
+        const int tau = (int)(it / a.count);
+        const int pidx = (int)(it - (long long)tau * a.count);
+        int s, t;
+        if constexpr (OBS == OBS_V) {
+            s = t = pidx;
+        } else {
+            s = a.pairs[2 * (long long)pidx];
+            t = a.pairs[2 * (long long)pidx + 1];
+        }
+        if (s < 0 || s >= a.nrv || t < 0 || t >= a.nrv) continue;   // malformed pair: skipped (uniform in the work-group)
+        const long long off = (long long)tau * V;
+        for (int q = threadIdx.x; q < V; q += blockDim.x) {
+            const int x = q % L;
+            cplx<R> va, vb;
+            obs_site<R, OBS>(a.eta, a.G, a.phix, a.N, s, t, off + q, off + q - x + (x + 1 == L ? 0 : x + 1), va, vb);
+            A[q] = va;
+            if constexpr (OBS != OBS_V) B[q] = vb;
+        }
+        __syncthreads();
+        stockham_2d<R>(A, T, tw, L, a.facs, a.nfac);
+        if constexpr (OBS != OBS_V) stockham_2d<R>(B, T, tw, L, a.facs, a.nfac);
+        const cplx<R>* Bh = (OBS == OBS_V) ? A : B;
+#pragma unroll
+        for (int m = 0; m < MAXK; ++m) {
+            const int k = threadIdx.x + m * 256;
+            if (k < V) {
+                const int ky = k / L, kx = k - ky * L;
+                const cplx<R> u = A[k];
+                const cplx<R> v = Bh[(ky ? L - ky : 0) * L + (kx ? L - kx : 0)];
+                pr[m] += (double)u.x * v.x - (double)u.y * v.y;
+                pi[m] += (double)u.x * v.y + (double)u.y * v.x;
+                if constexpr (OBS == OBS_V) {
+                    lr[m] += u.x;
+                    li[m] += u.y;
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int m = 0; m < MAXK; ++m) {
+        const int k = threadIdx.x + m * 256;
+        if (k < V) {
+            a.part[(long long)blockIdx.x * V + k] = make_double2(pr[m], pi[m]);
+            if constexpr (OBS == OBS_V) a.part[(long long)(gridDim.x + blockIdx.x) * V + k] = make_double2(lr[m], li[m]);
+        }
+    }
+}
+
+// LDS bytes of obs_fused_kernel for lattice L.
+template <typename R> inline size_t obs_fused_lds(int L) { return sizeof(cplx<R>) * ((size_t)L + 3 * (size_t)L * L); }
 
 }  // namespace qf

@@ -6,17 +6,20 @@ chain's boson field, and combines them like ``get_bond_bond_per_b2``.
 
 Every estimator is C(Delta) = (1/V) sum_r a(r - Delta) b(r), averaged over
 pairs and tau.  The reference evaluates it per batch of int(0.1 Nrv) pairs
-with three FFTs and a host loop (1405-1432).  Here, per chunk of pairs (sized
-to a memory budget, up to all of them):
+with three FFTs and a host loop (1405-1432).  Here the k-space sum
+S(k) = sum over pairs and slices of Ahat(k) Bhat(-k) is formed in FP64 and
+C(Delta) = fft2(S)(Delta) / V^2 / (pairs * Ltau): one transform of an [L, L]
+array per estimator.  Two ways to form S:
 
-  1. ``qf_obs_fields`` (HIP) writes a and b of every pair of the chunk;
-  2. a 2-D FFT of both (torch.fft -> rocFFT, forward, unnormalised);
-  3. ``qf_obs_kspace`` (HIP) adds Ahat(k) Bhat(-k) over every slice into FP64
-     per-work-group partials.
+* "fused" (default where the slices fit in LDS): one launch of
+  ``qf_obs_fused`` per estimator; each work-group builds a and b of one
+  (slice, pair) in LDS, transforms them with its own mixed-radix FFT and
+  accumulates in registers.  No intermediate touches HBM.
+* "rocfft": per chunk of pairs (sized to a memory budget) ``qf_obs_fields``
+  (HIP) writes a and b, torch.fft (rocFFT) transforms them and
+  ``qf_obs_kspace`` (HIP) adds the products into FP64 partials.
 
-After the last chunk the partials are reduced in order (``qf_obs_reduce``)
-and C(Delta) = fft2(S)(Delta) / V^2 / (pairs * Ltau): one transform of an
-[L, L] array per estimator.  The same-noise-vector terms each estimator adds
+The partials are reduced in a fixed order (``qf_obs_reduce``).  The same-noise-vector terms each estimator adds
 at one Delta come from ``qf_obs_diag``.  Inputs stay where they are (no
 rolled copies); all buffers are allocated once per shape, so a call can be
 captured in a graph after one warm-up call.
@@ -50,7 +53,8 @@ class BondBondEstimator:
     field buffer; two such buffers are held, and the transforms of a chunk take two more).  nwg: work-groups of the
     k-space accumulation (its FP64 partials are [nwg, L*L])."""
 
-    def __init__(self, L, Ltau, nrv, dtype=torch.complex64, device="cuda", pair_chunk=None, chunk_bytes=256 << 20, nwg=512):
+    def __init__(self, L, Ltau, nrv, dtype=torch.complex64, device="cuda", pair_chunk=None, chunk_bytes=256 << 20, nwg=512,
+                 method="auto"):
         if dtype not in _PREC:
             raise QfError(_lib.QF_ERR_INVALID, f"unsupported dtype {dtype}")
         if nrv < 2:
@@ -66,12 +70,22 @@ class BondBondEstimator:
         if pair_chunk is None:
             pair_chunk = max(1, chunk_bytes // (self.N * esize))
         self.chunk = int(min(pair_chunk, max(self.npairs, nrv)))
-        self.nwg = int(nwg)
-        self.nblk = 256
+        if method not in ("auto", "fused", "rocfft"):
+            raise ValueError(f"method {method!r}")
         dev = self.device
+        fused_nwg = 0
+        if method != "rocfft":
+            with torch.cuda.device(dev):
+                fused_nwg = _lib.lib().qf_obs_fused_grid(_PREC[dtype], L)
+            if method == "fused" and fused_nwg <= 0:
+                raise QfError(_lib.QF_ERR_UNSUPPORTED, f"no fused estimator kernel for L = {L}, {dtype}")
+        self.method = "fused" if fused_nwg > 0 else "rocfft"
+        self.nwg = int(fused_nwg if self.method == "fused" else nwg)
+        self.nblk = 256
         self.pairs = torch.combinations(torch.arange(nrv), r=2).to(torch.int32).reshape(-1).contiguous().to(dev)
-        self.A = torch.empty(self.chunk, self.N, dtype=dtype, device=dev)
-        self.B = torch.empty_like(self.A)
+        if self.method == "rocfft":
+            self.A = torch.empty(self.chunk, self.N, dtype=dtype, device=dev)
+            self.B = torch.empty_like(self.A)
         self.part = torch.empty(2 * self.nwg, self.V, dtype=torch.complex128, device=dev)
         self.S = torch.empty(2, self.V, dtype=torch.complex128, device=dev)
         self.dpart = torch.empty(self.nblk, 3, dtype=torch.complex128, device=dev)
@@ -103,6 +117,13 @@ class BondBondEstimator:
         prec = _PREC[self.dtype]
         lin = 1 if obs == _lib.QF_OBS_V else 0
         st = stream_handle()
+        if self.method == "fused":
+            call("qf_obs_fused", prec, obs, ptr(eta), ptr(G), ptr(phix), ptr(self.pairs), count, self.nrv, self.L, self.Ltau,
+                 ptr(self.part), self.nwg, st)
+            call("qf_obs_reduce", ptr(self.part), self.nwg, self.V, ptr(self.S[0]), st)
+            if lin:
+                call("qf_obs_reduce", ptr(self.part[self.nwg:]), self.nwg, self.V, ptr(self.S[1]), st)
+            return
         first = True
         for p0 in range(0, count, self.chunk):
             n = min(self.chunk, count - p0)

@@ -35,18 +35,20 @@ def _dev(eta, G, boson, dev, dtype):
     return (torch.from_numpy(eta).to(dev, dtype), torch.from_numpy(G).to(dev, dtype), torch.from_numpy(boson).to(dev, rd))
 
 
+@pytest.mark.parametrize("method", ["fused", "rocfft"])
 @pytest.mark.parametrize("dtype,tol", [(torch.complex128, 1e-10), (torch.complex64, 2e-5)])
 @pytest.mark.parametrize("L,Lt,nrv", [(4, 10, 5), (6, 12, 7)])
-def test_estimators_match_oracle(dev, dtype, tol, L, Lt, nrv):
+def test_estimators_match_oracle(dev, dtype, tol, L, Lt, nrv, method):
     from qed_fermion_amd.observables import BondBondEstimator
-    print(f"[obs] L={L} Lt={Lt} nrv={nrv} {dtype}", flush=True)
+    print(f"[obs] L={L} Lt={Lt} nrv={nrv} {dtype} {method}", flush=True)
     eta, G, boson = _inputs(L, Lt, nrv, seed=L * 100 + nrv)
     if dtype == torch.complex64:   # the oracle sees the rounded inputs
         eta = eta.astype(np.complex64).astype(np.complex128)
         G = G.astype(np.complex64).astype(np.complex128)
         boson = boson.astype(np.float32).astype(np.float64)
     e_d, G_d, b_d = _dev(eta, G, boson, dev, dtype)
-    est = BondBondEstimator(L, Lt, nrv, dtype=dtype, device=dev)
+    est = BondBondEstimator(L, Lt, nrv, dtype=dtype, device=dev, method=method)
+    assert est.method == method
     for name in ("T1", "T21", "T2", "T4"):
         got = est.T(name, e_d, G_d, b_d).cpu().numpy()
         want = OB.T(name, eta, G, boson, L, Lt)
@@ -66,14 +68,62 @@ def test_chunking_does_not_change_the_result(dev):
     L, Lt, nrv = 6, 16, 9
     eta, G, boson = _inputs(L, Lt, nrv, seed=5)
     e_d, G_d, b_d = _dev(eta, G, boson, dev, torch.complex128)
-    whole = BondBondEstimator(L, Lt, nrv, dtype=torch.complex128, device=dev)
-    small = BondBondEstimator(L, Lt, nrv, dtype=torch.complex128, device=dev, pair_chunk=4, nwg=7)
+    whole = BondBondEstimator(L, Lt, nrv, dtype=torch.complex128, device=dev, method="rocfft")
+    small = BondBondEstimator(L, Lt, nrv, dtype=torch.complex128, device=dev, pair_chunk=4, nwg=7, method="rocfft")
     for name in ("T1", "T21", "T2", "T4"):
         a = whole.T(name, e_d, G_d, b_d).cpu().numpy()
         b = small.T(name, e_d, G_d, b_d).cpu().numpy()
         assert _rel(b, a) < 1e-12, name
     a, b = whole.Tvv(e_d, G_d, b_d), small.Tvv(e_d, G_d, b_d)
     assert _rel(b[0].cpu().numpy(), a[0].cpu().numpy()) < 1e-12
+
+
+@pytest.mark.parametrize("L", [6, 10, 14, 22, 26, 36, 48])
+def test_fused_transform_radices(dev, L):
+    """The fused kernel's own FFT for lattices whose radices are 4, 2, 3, 5, 7 and larger primes
+    (11, 13 take the generic stage), complex64 against the oracle."""
+    from qed_fermion_amd.observables import BondBondEstimator
+    Lt, nrv = 4, 4
+    print(f"[obs] fused radices L={L}", flush=True)
+    eta, G, boson = _inputs(L, Lt, nrv, seed=L)
+    eta = eta.astype(np.complex64).astype(np.complex128)
+    G = G.astype(np.complex64).astype(np.complex128)
+    boson = boson.astype(np.float32).astype(np.float64)
+    e_d, G_d, b_d = _dev(eta, G, boson, dev, torch.complex64)
+    est = BondBondEstimator(L, Lt, nrv, dtype=torch.complex64, device=dev, method="fused")
+    assert _rel(est.T("T1", e_d, G_d, b_d).cpu().numpy(), OB.T("T1", eta, G, boson, L, Lt)) < 2e-5
+    tvv, tv = est.Tvv(e_d, G_d, b_d)
+    wvv, wv = OB.Tvv(eta, G, boson, L, Lt)
+    assert _rel(tvv.cpu().numpy(), wvv) < 2e-5 and _rel(tv.cpu().numpy(), wv) < 2e-5
+
+
+def test_fused_unsupported_sizes_fall_back(dev):
+    from qed_fermion_amd import _lib
+    from qed_fermion_amd.observables import BondBondEstimator
+    assert _lib.lib().qf_obs_fused_grid(_lib.QF_PREC_F64, 64) == 0      # complex128 slices of 64 x 64 exceed LDS
+    assert _lib.lib().qf_obs_fused_grid(_lib.QF_PREC_F32, 64) > 0
+    est = BondBondEstimator(64, 2, 2, dtype=torch.complex128, device=dev)
+    assert est.method == "rocfft"
+
+
+def test_fused_skips_malformed_pairs(dev):
+    from qed_fermion_amd import _lib
+    from qed_fermion_amd._lib import call, ptr, stream_handle
+    L, Lt, nrv = 6, 8, 3
+    N, V = L * L * Lt, L * L
+    eta, G, boson = _inputs(L, Lt, nrv, seed=4)
+    e_d, G_d, _ = _dev(eta, G, boson, dev, torch.complex128)
+    phix = torch.from_numpy(boson[0, 0].transpose(2, 1, 0).copy()).to(dev)
+    nwg = _lib.lib().qf_obs_fused_grid(_lib.QF_PREC_F64, L)
+    part = torch.empty(2 * nwg, V, dtype=torch.complex128, device=dev)
+    out = torch.empty(2, V, dtype=torch.complex128, device=dev)
+    for i, pl in enumerate(([0, 1], [0, 1, 1, nrv, -1, 2])):
+        pairs = torch.tensor(pl, dtype=torch.int32, device=dev)
+        call("qf_obs_fused", _lib.QF_PREC_F64, _lib.QF_OBS_T1, ptr(e_d), ptr(G_d), ptr(phix), ptr(pairs), len(pl) // 2, nrv, L, Lt,
+             ptr(part), nwg, stream_handle())
+        call("qf_obs_reduce", ptr(part), nwg, V, ptr(out[i]), stream_handle())
+    torch.cuda.synchronize()
+    assert _rel(out[1].cpu().numpy(), out[0].cpu().numpy()) < 1e-13
 
 
 def test_malformed_pair_list_gives_zero_fields(dev):
