@@ -178,17 +178,20 @@ int qf_hamiltonian(int prec, const double* x, const double* p, const void* psi, 
  * into per-work-group partials, and one transform of the reduced [L, L]
  * array gives C.  eta and G (= M^{-1} eta) are [nrv][Ltau][L][L] complex
  * (complex64 for QF_PREC_F32, complex128 for QF_PREC_F64); phix [Ltau][L][L]
- * is the chain's x-link angle in the matching real type. */
+ * is the chain's x-link angle in the matching real type and U = e^{i phix}
+ * its phase field (qf_obs_phase), complex of the same precision. */
 #define QF_OBS_T1 0
 #define QF_OBS_T21 1
 #define QF_OBS_T2 2
 #define QF_OBS_T4 3
 #define QF_OBS_V 4
+/* U[i] = exp(i phix[i]) for i < n (accurate sincos). */
+int qf_obs_phase(int prec, const void* phix, void* U, long long n, void* stream);
 /* Fields a -> A[j], b -> B[j] for pairs j = p0 .. p0+np-1 of pairs[2*j], pairs[2*j+1]
  * (s < s', torch.combinations order; entries outside [0, nrv) give zero fields).
  * QF_OBS_V writes v_s of Tvv for noise vectors p0 .. p0+np-1 into A (pairs, B unused).
  * A, B: [np][Ltau*L*L]. */
-int qf_obs_fields(int prec, int obs, const void* eta, const void* G, const void* phix, const int* pairs, int p0, int np, int nrv, int L,
+int qf_obs_fields(int prec, int obs, const void* eta, const void* G, const void* U, const int* pairs, int p0, int np, int nrv, int L,
                   int Ltau, void* A, void* B, void* stream);
 /* part[w][k] (+)= sum over slices sl = w, w+nwg, ... < nslices of A_hat[sl][k] * B_hat[sl][-k]
  * (double2, [nwg][L*L]); with lin also part[nwg + w][k] (+)= sum A_hat[sl][k].
@@ -211,7 +214,7 @@ int qf_obs_diag(int prec, const void* eta, const void* G, const void* phix, int 
  * vectors, <= nrv).  Returns QF_ERR_UNSUPPORTED where the slices do not fit
  * in LDS (complex128 with L > 58, any L > 78). */
 int qf_obs_fused_grid(int prec, int L);   /* work-groups that fill the device; 0 = unsupported */
-int qf_obs_fused(int prec, int obs, const void* eta, const void* G, const void* phix, const int* pairs, int count, int nrv, int L, int Ltau,
+int qf_obs_fused(int prec, int obs, const void* eta, const void* G, const void* U, const int* pairs, int count, int nrv, int L, int Ltau,
                  void* part, int nwg, void* stream);
 
 #ifdef __cplusplus

@@ -78,6 +78,7 @@ SIGNATURES = {
     "qf_momentum_kick": (_I, [_P, _P, _P, _D, _P, _I, _I, _I, _P]),
     "qf_hamiltonian_workspace_bytes": (_SZ, [_I]),
     "qf_hamiltonian": (_I, [_I, _P, _P, _P, _P, _P, _P, _I, _I, _I, _D, _D, _I, _D, _P, _SZ, _P]),
+    "qf_obs_phase": (_I, [_I, _P, _P, _LL, _P]),
     "qf_obs_fields": (_I, [_I, _I, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P, _P, _P]),
     "qf_obs_kspace": (_I, [_I, _P, _P, _LL, _I, _I, _I, _P, _I, _P]),
     "qf_obs_reduce": (_I, [_P, _I, _I, _P, _P]),

@@ -566,6 +566,36 @@ bool obs_factor(int L, unsigned long long& facs, int& nfac) {
 }
 
 
+// Launch shape of the fused estimator kernel for (R, estimator, L):
+// kObsThreads threads, MAXK sites per thread, DUAL when [A | B] and two ping-pong
+// arrays fit in LDS.  Calls f(kernel, lds); QF_ERR_UNSUPPORTED when nothing fits.
+template <typename R, int O, typename F> int obs_fused_pick(int L, F&& f) {
+    const int V = L * L;
+    constexpr size_t kLds = 160 * 1024;
+    const bool dual = O != OBS_V && obs_fused_lds<R>(L, O, true) <= kLds;
+    const size_t lds = obs_fused_lds<R>(L, O, dual);
+    if (lds > kLds || V > 6 * 1024) return fail(QF_ERR_UNSUPPORTED, "fused estimator kernel: %zu bytes of LDS at L = %d", lds, L);
+    auto by_k = [&](auto dual_tag) -> int {
+        constexpr bool D = decltype(dual_tag)::value;
+        if (V <= 1024) return f(obs_fused_kernel<R, O, kObsThreads, 2, D>, lds);
+        if (V <= 2048) return f(obs_fused_kernel<R, O, kObsThreads, 4, D>, lds);
+        if (V <= 4096) return f(obs_fused_kernel<R, O, kObsThreads, 8, D>, lds);
+        return f(obs_fused_kernel<R, O, kObsThreads, 12, D>, lds);
+    };
+    if constexpr (O == OBS_V) return by_k(std::false_type{});
+    else return dual ? by_k(std::true_type{}) : by_k(std::false_type{});
+}
+
+template <typename R, typename F> int obs_fused_dispatch(int obs, int L, F&& f) {
+    switch (obs) {
+        case QF_OBS_T1: return obs_fused_pick<R, OBS_T1>(L, f);
+        case QF_OBS_T21: return obs_fused_pick<R, OBS_T21>(L, f);
+        case QF_OBS_T2: return obs_fused_pick<R, OBS_T2>(L, f);
+        case QF_OBS_T4: return obs_fused_pick<R, OBS_T4>(L, f);
+        default: return obs_fused_pick<R, OBS_V>(L, f);
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1095,9 +1125,23 @@ int qf_hamiltonian(int prec, const double* x, const double* p, const void* psi, 
 }
 
 // ---------------------------------------------------------------- bond-bond estimators
-int qf_obs_fields(int prec, int obs, const void* eta, const void* G, const void* phix, const int* pairs, int p0, int np, int nrv, int L,
+int qf_obs_phase(int prec, const void* phix, void* U, long long n, void* stream) {
+    if (!phix || !U) return fail(QF_ERR_INVALID, "null pointer");
+    if (n < 1) return fail(QF_ERR_INVALID, "bad sizes");
+    const dim3 grid((unsigned)std::min<long long>((n + 255) / 256, 16384)), blk(256);
+    if (prec == QF_PREC_F32)
+        hipLaunchKernelGGL((obs_phase_kernel<float>), grid, blk, 0, (hipStream_t)stream, (const float*)phix, (float2*)U, n);
+    else if (prec == QF_PREC_F64)
+        hipLaunchKernelGGL((obs_phase_kernel<double>), grid, blk, 0, (hipStream_t)stream, (const double*)phix, (double2*)U, n);
+    else
+        return fail(QF_ERR_INVALID, "prec %d", prec);
+    QF_LAUNCHED();
+    return QF_OK;
+}
+
+int qf_obs_fields(int prec, int obs, const void* eta, const void* G, const void* U, const int* pairs, int p0, int np, int nrv, int L,
                   int Ltau, void* A, void* B, void* stream) {
-    if (!eta || !G || !phix || !A) return fail(QF_ERR_INVALID, "null pointer");
+    if (!eta || !G || !U || !A) return fail(QF_ERR_INVALID, "null pointer");
     if (obs < QF_OBS_T1 || obs > QF_OBS_V) return fail(QF_ERR_INVALID, "estimator %d", obs);
     if (obs != QF_OBS_V && (!pairs || !B)) return fail(QF_ERR_INVALID, "pair estimators need pairs and B");
     if (L < 2 || Ltau < 1 || nrv < 1 || np < 1 || p0 < 0) return fail(QF_ERR_INVALID, "bad sizes");
@@ -1108,7 +1152,7 @@ int qf_obs_fields(int prec, int obs, const void* eta, const void* G, const void*
     auto run = [&](auto tag) -> int {
         using R = decltype(tag);
         ObsArgs<R> a{};
-        a.eta = (const cplx<R>*)eta; a.G = (const cplx<R>*)G; a.phix = (const R*)phix; a.pairs = pairs;
+        a.eta = (const cplx<R>*)eta; a.G = (const cplx<R>*)G; a.U = (const cplx<R>*)U; a.pairs = pairs;
         a.p0 = p0; a.np = np; a.nrv = nrv; a.L = L; a.N = N; a.A = (cplx<R>*)A; a.B = (cplx<R>*)B;
         switch (obs) {
             case QF_OBS_T1: hipLaunchKernelGGL((obs_fields_kernel<R, OBS_T1>), grid, blk, 0, s, a); break;
@@ -1170,44 +1214,37 @@ int qf_obs_diag(int prec, const void* eta, const void* G, const void* phix, int 
 
 int qf_obs_fused_grid(int prec, int L) {
     if (L < 2 || L > 78) return 0;
-    unsigned long long f;
+    unsigned long long fc;
     int n;
-    if (!obs_factor(L, f, n)) return 0;
+    if (!obs_factor(L, fc, n)) return 0;
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1) return 0;
-    const int V = L * L;
     // resident work-groups per CU of the least-occupying estimator kernel at this L
-    auto occ = [&](auto tag) -> int {
+    int best = 1 << 30;
+    auto occ = [&](auto tag) {
         using R = decltype(tag);
-        const size_t lds = obs_fused_lds<R>(L);
-        if (lds > 160 * 1024) return 0;
-        int best = 1 << 30;
-        auto one = [&](auto kernel) {
-            int b = 0;
-            if (set_lds(kernel, lds) == QF_OK && hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, 256, lds) == hipSuccess)
+        for (int obs = QF_OBS_T1; obs <= QF_OBS_V; ++obs) {
+            const int rc = obs_fused_dispatch<R>(obs, L, [&](auto kernel, size_t lds) -> int {
+                int b = 0;
+                if (set_lds(kernel, lds) != QF_OK || hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, kObsThreads, lds) != hipSuccess)
+                    b = 0;
                 best = std::min(best, b);
-            else
-                best = 0;
-        };
-        auto all = [&](auto k4, auto k8, auto k16, auto k24) {
-            if (V <= 1024) one(k4); else if (V <= 2048) one(k8); else if (V <= 4096) one(k16); else one(k24);
-        };
-        all(obs_fused_kernel<R, OBS_T1, 4>, obs_fused_kernel<R, OBS_T1, 8>, obs_fused_kernel<R, OBS_T1, 16>, obs_fused_kernel<R, OBS_T1, 24>);
-        all(obs_fused_kernel<R, OBS_T21, 4>, obs_fused_kernel<R, OBS_T21, 8>, obs_fused_kernel<R, OBS_T21, 16>,
-            obs_fused_kernel<R, OBS_T21, 24>);
-        all(obs_fused_kernel<R, OBS_T2, 4>, obs_fused_kernel<R, OBS_T2, 8>, obs_fused_kernel<R, OBS_T2, 16>, obs_fused_kernel<R, OBS_T2, 24>);
-        all(obs_fused_kernel<R, OBS_T4, 4>, obs_fused_kernel<R, OBS_T4, 8>, obs_fused_kernel<R, OBS_T4, 16>, obs_fused_kernel<R, OBS_T4, 24>);
-        all(obs_fused_kernel<R, OBS_V, 4>, obs_fused_kernel<R, OBS_V, 8>, obs_fused_kernel<R, OBS_V, 16>, obs_fused_kernel<R, OBS_V, 24>);
-        return best == (1 << 30) ? 0 : best;
+                return QF_OK;
+            });
+            if (rc) best = 0;
+        }
     };
-    const int b = prec == QF_PREC_F32 ? occ(float{}) : prec == QF_PREC_F64 ? occ(double{}) : 0;
-    return std::min(b * ncu, 65535);
+    if (prec == QF_PREC_F32) occ(float{});
+    else if (prec == QF_PREC_F64) occ(double{});
+    else return 0;
+    if (best <= 0 || best == (1 << 30)) return 0;
+    return std::min(best * ncu, 65535);
 }
 
-int qf_obs_fused(int prec, int obs, const void* eta, const void* G, const void* phix, const int* pairs, int count, int nrv, int L, int Ltau,
+int qf_obs_fused(int prec, int obs, const void* eta, const void* G, const void* U, const int* pairs, int count, int nrv, int L, int Ltau,
                  void* part, int nwg, void* stream) {
-    if (!eta || !G || !phix || !part) return fail(QF_ERR_INVALID, "null pointer");
+    if (!eta || !G || !U || !part) return fail(QF_ERR_INVALID, "null pointer");
     if (obs < QF_OBS_T1 || obs > QF_OBS_V) return fail(QF_ERR_INVALID, "estimator %d", obs);
     if (obs != QF_OBS_V && !pairs) return fail(QF_ERR_INVALID, "pair estimators need pairs");
     if (L < 2 || Ltau < 1 || nrv < 1 || count < 1 || nwg < 1 || nwg > 65535) return fail(QF_ERR_INVALID, "bad sizes");
@@ -1217,36 +1254,19 @@ int qf_obs_fused(int prec, int obs, const void* eta, const void* G, const void* 
     int nfac;
     if (!obs_factor(L, facs, nfac)) return fail(QF_ERR_UNSUPPORTED, "fused estimator kernel: L = %d has too many factors", L);
     hipStream_t s = (hipStream_t)stream;
-    const int V = L * L;
     auto run = [&](auto tag) -> int {
         using R = decltype(tag);
-        const size_t lds = obs_fused_lds<R>(L);
-        if (lds > 160 * 1024) return fail(QF_ERR_UNSUPPORTED, "fused estimator kernel: %zu bytes of LDS at L = %d", lds, L);
         ObsFusedArgs<R> a{};
-        a.eta = (const cplx<R>*)eta; a.G = (const cplx<R>*)G; a.phix = (const R*)phix; a.pairs = pairs;
-        a.count = count; a.nrv = nrv; a.L = L; a.Lt = Ltau; a.N = (long long)V * Ltau; a.nfac = nfac; a.facs = facs;
+        a.eta = (const cplx<R>*)eta; a.G = (const cplx<R>*)G; a.U = (const cplx<R>*)U; a.pairs = pairs;
+        a.count = count; a.nrv = nrv; a.L = L; a.Lt = Ltau; a.N = (long long)L * L * Ltau; a.nfac = nfac; a.facs = facs;
         a.part = (double2*)part;
-        auto go = [&](auto kernel) -> int {
+        return obs_fused_dispatch<R>(obs, L, [&](auto kernel, size_t lds) -> int {
             int rc = set_lds(kernel, lds);
             if (rc) return rc;
-            hipLaunchKernelGGL(kernel, dim3(nwg), dim3(256), lds, s, a);
+            hipLaunchKernelGGL(kernel, dim3(nwg), dim3(kObsThreads), lds, s, a);
             QF_LAUNCHED();
             return QF_OK;
-        };
-        auto by_k = [&](auto obs_tag) -> int {
-            constexpr int O = decltype(obs_tag)::value;
-            if (V <= 1024) return go(obs_fused_kernel<R, O, 4>);
-            if (V <= 2048) return go(obs_fused_kernel<R, O, 8>);
-            if (V <= 4096) return go(obs_fused_kernel<R, O, 16>);
-            return go(obs_fused_kernel<R, O, 24>);
-        };
-        switch (obs) {
-            case QF_OBS_T1: return by_k(std::integral_constant<int, OBS_T1>{});
-            case QF_OBS_T21: return by_k(std::integral_constant<int, OBS_T21>{});
-            case QF_OBS_T2: return by_k(std::integral_constant<int, OBS_T2>{});
-            case QF_OBS_T4: return by_k(std::integral_constant<int, OBS_T4>{});
-            default: return by_k(std::integral_constant<int, OBS_V>{});
-        }
+        });
     };
     if (prec == QF_PREC_F32) return run(float{});
     if (prec == QF_PREC_F64) return run(double{});

@@ -21,7 +21,7 @@ enum { OBS_T1 = 0, OBS_T21 = 1, OBS_T2 = 2, OBS_T4 = 3, OBS_V = 4 };
 template <typename R> struct ObsArgs {
     const cplx<R>* eta;   // noise vectors (conjugated here: e = eta*)
     const cplx<R>* G;     // G_eta
-    const R* phix;        // x-link angles of the chain
+    const cplx<R>* U;     // e^{i phix}: x-link phases of the chain
     const int* pairs;     // [2 * npairs] (s, s'), pair estimators only
     int p0, np;           // pairs (OBS_V: noise vectors) [p0, p0 + np) of this launch
     int nrv, L;
@@ -52,13 +52,13 @@ template <> __device__ __forceinline__ double2 expi<double>(double p) {
 //   T2  a = (e_s(r) G_t(r+) + e_t(r) G_s(r+)) U/2     b = (G_s(r) e_t(r+) + G_t(r) e_s(r+)) Ub/2  (1530-1544)
 //   T4  a = e_s(r+) G_t(r) Ub                         b = G_s(r) e_t(r+) Ub           (1587-1593)
 //   V   a = G_s(r+) e_s(r) U + G_s(r) e_s(r+) Ub                                     (1631-1636)
-// site indexes a vector (tau * V + y * L + x), xp is its x-neighbour.
+// site indexes a vector (tau * V + y * L + x), xp is its x-neighbour; Uf = e^{i phix}.
 template <typename R, int OBS>
-__device__ __forceinline__ void obs_site(const cplx<R>* eta, const cplx<R>* G, const R* phix, long long N, int s, int t, long long site,
+__device__ __forceinline__ void obs_site(const cplx<R>* eta, const cplx<R>* G, const cplx<R>* Uf, long long N, int s, int t, long long site,
                                          long long xp, cplx<R>& va, cplx<R>& vb) {
     const cplx<R>* es = eta + (long long)s * N;
     const cplx<R>* gs = G + (long long)s * N;
-    const cplx<R> U = expi<R>(phix[site]);
+    const cplx<R> U = Uf[site];
     const cplx<R> Ub = cconj<R>(U);
     const cplx<R> e0 = cconj<R>(es[site]), e1 = cconj<R>(es[xp]);
     const cplx<R> g0 = gs[site], g1 = gs[xp];
@@ -86,6 +86,12 @@ __device__ __forceinline__ void obs_site(const cplx<R>* eta, const cplx<R>* G, c
     }
 }
 
+// U[i] = e^{i phix[i]} (accurate sincos), once per estimator call.
+template <typename R>
+__global__ void __launch_bounds__(256) obs_phase_kernel(const R* phix, cplx<R>* U, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) U[i] = expi<R>(phix[i]);
+}
+
 // Fields of every (pair, site) of the launch into A, B (rocFFT path).
 template <typename R, int OBS>
 __global__ void __launch_bounds__(256) obs_fields_kernel(ObsArgs<R> a) {
@@ -110,7 +116,7 @@ __global__ void __launch_bounds__(256) obs_fields_kernel(ObsArgs<R> a) {
             continue;
         }
         cplx<R> va, vb;
-        obs_site<R, OBS>(a.eta, a.G, a.phix, a.N, s, t, site, xp, va, vb);
+        obs_site<R, OBS>(a.eta, a.G, a.U, a.N, s, t, site, xp, va, vb);
         a.A[i] = va;
         if constexpr (OBS != OBS_V) a.B[i] = vb;
     }
@@ -218,12 +224,13 @@ __global__ void __launch_bounds__(256) obs_diag_kernel(const cplx<R>* eta, const
 // columns, and adds Ahat(k) Bhat(-k) to FP64 registers.  Only the inputs (from
 // L2) and one partial per work-group touch global memory.
 
-constexpr int kObsMaxFac = 8;   // radices of one transform (L <= 78 needs at most 4)
+constexpr int kObsMaxFac = 8;     // radices of one transform (L <= 78 needs at most 4)
+constexpr int kObsThreads = 512;  // threads of a fused work-group (up to 256 VGPRs per lane, no spills)
 
 template <typename R> struct ObsFusedArgs {
     const cplx<R>* eta;
     const cplx<R>* G;
-    const R* phix;
+    const cplx<R>* U;           // e^{i phix}
     const int* pairs;
     int count;                  // pairs (OBS_V: noise vectors)
     int nrv, L, Lt;
@@ -238,17 +245,56 @@ __device__ __forceinline__ cplx<R> tw_mul(cplx<R> u, const cplx<R>* tw, int e) {
     return e ? cmul<R>(u, tw[e]) : u;
 }
 
-// One Stockham stage of compile-time radix P over L lines of length L
-// (element i of line l at l * ls + i * es); tw[m] = e^{-2 pi i m / L}.
+// floor(a / d) for 0 <= a < 2^16 and 1 <= d <= 2^8 from rd = 1.0f / d: the
+// fractional part of (a + 0.5) / d is at least 0.5 / d > 1e-3, far above the
+// float rounding of the product, so the truncation is exact.  Replaces the
+// ~20-instruction integer division in the FFT's index arithmetic.
+__device__ __forceinline__ int fdiv(int a, float rd) { return (int)(((float)a + 0.5f) * rd); }
+
+// First element of line l: rows (col = false) of the [n][L][L] arrays are
+// lines at l*L, stride 1; columns are line l = array*L + x at
+// array*L*L + x, stride L.
+__device__ __forceinline__ int line_base(int l, int L, float rL, bool col) {
+    if (!col) return l * L;
+    const int arr = fdiv(l, rL);
+    return arr * L * L + (l - arr * L);
+}
+
+// One Stockham stage of compile-time radix P over nlines lines of length L;
+// tw[m] = e^{-2 pi i m / L}.
 template <typename R, int P>
-__device__ __forceinline__ void stockham_stage(const cplx<R>* src, cplx<R>* dst, const cplx<R>* tw, int L, int Ns, int ls, int es) {
-    const int nb = L / P, total = L * nb, tstep = L / (Ns * P), wstep = L / P;
+__device__ __forceinline__ void stockham_stage(const cplx<R>* src, cplx<R>* dst, const cplx<R>* tw, int L, int Ns, int nlines, bool col) {
+    const int nb = L / P, total = nlines * nb, tstep = L / (Ns * P), wstep = L / P, es = col ? L : 1;
+    const float rnb = 1.0f / nb, rnl = 1.0f / nlines, rNs = 1.0f / Ns, rL = 1.0f / L;
     for (int b = threadIdx.x; b < total; b += blockDim.x) {
-        const int line = b / nb, j = b - line * nb, k = j % Ns;
-        const int sb = line * ls;
+        // rows: neighbouring threads take neighbouring butterflies of a row; columns: the same
+        // butterfly of neighbouring columns, so LDS accesses stay at unit stride either way
+        int line, j;
+        if (col) {
+            j = fdiv(b, rnl);
+            line = b - j * nlines;
+        } else {
+            line = fdiv(b, rnb);
+            j = b - line * nb;
+        }
+        const int k = j - fdiv(j, rNs) * Ns;
+        const int sb = line_base(line, L, rL, col);
         cplx<R> v[P];
+        // twiddles w^r, w = tw[k * tstep] (k * tstep < L), by repeated multiplication: one LDS
+        // read per butterfly instead of P - 1 scattered ones
+        v[0] = src[sb + j * es];
+        if (k == 0) {
 #pragma unroll
-        for (int r = 0; r < P; ++r) v[r] = tw_mul<R>(src[sb + (j + r * nb) * es], tw, (r * k * tstep) % L);
+            for (int r = 1; r < P; ++r) v[r] = src[sb + (j + r * nb) * es];
+        } else {
+            const cplx<R> w1 = tw[k * tstep];
+            cplx<R> w = w1;
+#pragma unroll
+            for (int r = 1; r < P; ++r) {
+                v[r] = cmul<R>(src[sb + (j + r * nb) * es], w);
+                if (r + 1 < P) w = cmul<R>(w, w1);
+            }
+        }
         const int d0 = sb + ((j - k) * P + k) * es, dq = Ns * es;
         if constexpr (P == 2) {
             dst[d0] = cadd<R>(v[0], v[1]);
@@ -281,42 +327,55 @@ __device__ __forceinline__ void stockham_stage(const cplx<R>* src, cplx<R>* dst,
 
 // The same for any radix P (larger primes): inputs are re-read from LDS per output.
 template <typename R>
-__device__ __forceinline__ void stockham_stage_any(const cplx<R>* src, cplx<R>* dst, const cplx<R>* tw, int L, int P, int Ns, int ls,
-                                                   int es) {
-    const int nb = L / P, total = L * nb, tstep = L / (Ns * P), wstep = L / P;
+__device__ __forceinline__ void stockham_stage_any(const cplx<R>* src, cplx<R>* dst, const cplx<R>* tw, int L, int P, int Ns, int nlines,
+                                                   bool col) {
+    const int nb = L / P, total = nlines * nb, tstep = L / (Ns * P), wstep = L / P, es = col ? L : 1;
+    const float rnb = 1.0f / nb, rnl = 1.0f / nlines, rNs = 1.0f / Ns, rL = 1.0f / L;
     for (int b = threadIdx.x; b < total; b += blockDim.x) {
-        const int line = b / nb, j = b - line * nb, k = j % Ns;
-        const int sb = line * ls;
+        int line, j;
+        if (col) {
+            j = fdiv(b, rnl);
+            line = b - j * nlines;
+        } else {
+            line = fdiv(b, rnb);
+            j = b - line * nb;
+        }
+        const int k = j - fdiv(j, rNs) * Ns;
+        const int sb = line_base(line, L, rL, col);
         const int d0 = sb + ((j - k) * P + k) * es;
         for (int q = 0; q < P; ++q) {
             cplx<R> y = mk<R>(0, 0);
+            int qr = 0;                                   // (q * r) mod P
             for (int r = 0; r < P; ++r) {
-                const cplx<R> u = tw_mul<R>(src[sb + (j + r * nb) * es], tw, (r * k * tstep) % L);
-                y = cadd<R>(y, tw_mul<R>(u, tw, ((q * r) % P) * wstep));
+                const cplx<R> u = tw_mul<R>(src[sb + (j + r * nb) * es], tw, r * k * tstep);
+                y = cadd<R>(y, tw_mul<R>(u, tw, qr * wstep));
+                qr += q;
+                if (qr >= P) qr -= P;
             }
             dst[d0 + q * Ns * es] = y;
         }
     }
 }
 
-// Forward 2-D transform of X ([L][L], in LDS) in place, T as the ping-pong
-// buffer: rows, then columns; 2 * nfac stages leave the result in X.
+// Forward 2-D transforms of the n arrays [n][L][L] at X (LDS) in place, T
+// ([n][L][L]) as the ping-pong buffer: rows of all arrays, then columns;
+// 2 * nfac stages leave the result in X.
 template <typename R>
-__device__ void stockham_2d(cplx<R>* X, cplx<R>* T, const cplx<R>* tw, int L, unsigned long long facs, int nfac) {
+__device__ void stockham_2d(cplx<R>* X, cplx<R>* T, const cplx<R>* tw, int L, int n, unsigned long long facs, int nfac) {
     cplx<R>* src = X;
     cplx<R>* dst = T;
     for (int pass = 0; pass < 2; ++pass) {
-        const int ls = pass ? 1 : L, es = pass ? L : 1;
+        const bool col = pass != 0;
         int Ns = 1;
         for (int f = 0; f < nfac; ++f) {
             const int P = (int)((facs >> (8 * f)) & 0xFF);
             switch (P) {
-                case 2: stockham_stage<R, 2>(src, dst, tw, L, Ns, ls, es); break;
-                case 3: stockham_stage<R, 3>(src, dst, tw, L, Ns, ls, es); break;
-                case 4: stockham_stage<R, 4>(src, dst, tw, L, Ns, ls, es); break;
-                case 5: stockham_stage<R, 5>(src, dst, tw, L, Ns, ls, es); break;
-                case 7: stockham_stage<R, 7>(src, dst, tw, L, Ns, ls, es); break;
-                default: stockham_stage_any<R>(src, dst, tw, L, P, Ns, ls, es); break;
+                case 2: stockham_stage<R, 2>(src, dst, tw, L, Ns, n * L, col); break;
+                case 3: stockham_stage<R, 3>(src, dst, tw, L, Ns, n * L, col); break;
+                case 4: stockham_stage<R, 4>(src, dst, tw, L, Ns, n * L, col); break;
+                case 5: stockham_stage<R, 5>(src, dst, tw, L, Ns, n * L, col); break;
+                case 7: stockham_stage<R, 7>(src, dst, tw, L, Ns, n * L, col); break;
+                default: stockham_stage_any<R>(src, dst, tw, L, P, Ns, n * L, col); break;
             }
             __syncthreads();
             Ns *= P;
@@ -328,16 +387,19 @@ __device__ void stockham_2d(cplx<R>* X, cplx<R>* T, const cplx<R>* tw, int L, un
 }
 
 // part[w][k] = sum over the work-group's items of Ahat(k) Bhat(-k); for OBS_V
-// (Bhat = Ahat) also part[gridDim.x + w][k] = sum Ahat(k).  MAXK >= L*L / 256.
-template <typename R, int OBS, int MAXK>
-__global__ void __launch_bounds__(256) obs_fused_kernel(ObsFusedArgs<R> a) {
+// (Bhat = Ahat) also part[gridDim.x + w][k] = sum Ahat(k).  NT threads,
+// MAXK * NT >= L*L.  DUAL: A and B are transformed together as one batch of
+// 2L lines (LDS [A | B] + [T | T]); otherwise one after the other through T.
+template <typename R, int OBS, int NT, int MAXK, bool DUAL>
+__global__ void __launch_bounds__(NT) obs_fused_kernel(ObsFusedArgs<R> a) {
     extern __shared__ __align__(16) unsigned char obs_lds[];
     const int L = a.L, V = L * L;
+    const float rL = 1.0f / L;
     cplx<R>* tw = reinterpret_cast<cplx<R>*>(obs_lds);
     cplx<R>* A = tw + L;
     cplx<R>* B = A + V;
-    cplx<R>* T = B + V;
-    for (int j = threadIdx.x; j < L; j += blockDim.x) {
+    cplx<R>* T = (OBS == OBS_V) ? B : B + V;    // OBS_V needs no B
+    for (int j = threadIdx.x; j < L; j += NT) {
         double sn, cs;
         sincos(-6.283185307179586476925 * j / L, &sn, &cs);
         tw[j] = mk<R>((R)cs, (R)sn);
@@ -359,22 +421,32 @@ __global__ void __launch_bounds__(256) obs_fused_kernel(ObsFusedArgs<R> a) {
         }
         if (s < 0 || s >= a.nrv || t < 0 || t >= a.nrv) continue;   // malformed pair: skipped (uniform in the work-group)
         const long long off = (long long)tau * V;
-        for (int q = threadIdx.x; q < V; q += blockDim.x) {
-            const int x = q % L;
-            cplx<R> va, vb;
-            obs_site<R, OBS>(a.eta, a.G, a.phix, a.N, s, t, off + q, off + q - x + (x + 1 == L ? 0 : x + 1), va, vb);
-            A[q] = va;
-            if constexpr (OBS != OBS_V) B[q] = vb;
+#pragma unroll
+        for (int m = 0; m < MAXK; ++m) {
+            const int q = threadIdx.x + m * NT;
+            if (q < V) {
+                const int x = q - fdiv(q, rL) * L;
+                cplx<R> va, vb;
+                obs_site<R, OBS>(a.eta, a.G, a.U, a.N, s, t, off + q, off + q - x + (x + 1 == L ? 0 : x + 1), va, vb);
+                A[q] = va;
+                if constexpr (OBS != OBS_V) B[q] = vb;
+            }
         }
         __syncthreads();
-        stockham_2d<R>(A, T, tw, L, a.facs, a.nfac);
-        if constexpr (OBS != OBS_V) stockham_2d<R>(B, T, tw, L, a.facs, a.nfac);
+        if constexpr (OBS == OBS_V) {
+            stockham_2d<R>(A, T, tw, L, 1, a.facs, a.nfac);
+        } else if constexpr (DUAL) {
+            stockham_2d<R>(A, T, tw, L, 2, a.facs, a.nfac);
+        } else {
+            stockham_2d<R>(A, T, tw, L, 1, a.facs, a.nfac);
+            stockham_2d<R>(B, T, tw, L, 1, a.facs, a.nfac);
+        }
         const cplx<R>* Bh = (OBS == OBS_V) ? A : B;
 #pragma unroll
         for (int m = 0; m < MAXK; ++m) {
-            const int k = threadIdx.x + m * 256;
+            const int k = threadIdx.x + m * NT;
             if (k < V) {
-                const int ky = k / L, kx = k - ky * L;
+                const int ky = fdiv(k, rL), kx = k - ky * L;
                 const cplx<R> u = A[k];
                 const cplx<R> v = Bh[(ky ? L - ky : 0) * L + (kx ? L - kx : 0)];
                 pr[m] += (double)u.x * v.x - (double)u.y * v.y;
@@ -389,7 +461,7 @@ __global__ void __launch_bounds__(256) obs_fused_kernel(ObsFusedArgs<R> a) {
     }
 #pragma unroll
     for (int m = 0; m < MAXK; ++m) {
-        const int k = threadIdx.x + m * 256;
+        const int k = threadIdx.x + m * NT;
         if (k < V) {
             a.part[(long long)blockIdx.x * V + k] = make_double2(pr[m], pi[m]);
             if constexpr (OBS == OBS_V) a.part[(long long)(gridDim.x + blockIdx.x) * V + k] = make_double2(lr[m], li[m]);
@@ -397,7 +469,12 @@ __global__ void __launch_bounds__(256) obs_fused_kernel(ObsFusedArgs<R> a) {
     }
 }
 
-// LDS bytes of obs_fused_kernel for lattice L.
-template <typename R> inline size_t obs_fused_lds(int L) { return sizeof(cplx<R>) * ((size_t)L + 3 * (size_t)L * L); }
+// LDS bytes of obs_fused_kernel: twiddles, A, B (pair estimators) and the
+// ping-pong buffer (2 arrays when DUAL).
+template <typename R> inline size_t obs_fused_lds(int L, int obs, bool dual) {
+    const size_t V = (size_t)L * L;
+    const size_t arrays = obs == OBS_V ? 2 : (dual ? 4 : 3);
+    return sizeof(cplx<R>) * ((size_t)L + arrays * V);
+}
 
 }  // namespace qf

@@ -90,6 +90,7 @@ class BondBondEstimator:
         self.S = torch.empty(2, self.V, dtype=torch.complex128, device=dev)
         self.dpart = torch.empty(self.nblk, 3, dtype=torch.complex128, device=dev)
         self.dsum = torch.empty(3, dtype=torch.complex128, device=dev)
+        self.U = torch.empty(self.N, dtype=dtype, device=dev)          # e^{i phix} of the current call
 
     # ------------------------------------------------------------ helpers
     def _inputs(self, eta, G_eta, boson):
@@ -104,6 +105,7 @@ class BondBondEstimator:
             if not t.is_cuda:
                 raise QfError(_lib.QF_ERR_INVALID, "tensor is not on the GPU; the HIP path has no CPU fallback")
         phix = boson.reshape(-1, 2, L, L, Lt)[0, 0].permute(2, 1, 0).to(_REAL[self.dtype]).contiguous()   # [tau, y, x]
+        call("qf_obs_phase", _PREC[self.dtype], ptr(phix), ptr(self.U), self.N, stream_handle())
         return eta.contiguous(), G_eta.contiguous(), phix
 
     def _fft(self, src, n):
@@ -118,7 +120,7 @@ class BondBondEstimator:
         lin = 1 if obs == _lib.QF_OBS_V else 0
         st = stream_handle()
         if self.method == "fused":
-            call("qf_obs_fused", prec, obs, ptr(eta), ptr(G), ptr(phix), ptr(self.pairs), count, self.nrv, self.L, self.Ltau,
+            call("qf_obs_fused", prec, obs, ptr(eta), ptr(G), ptr(self.U), ptr(self.pairs), count, self.nrv, self.L, self.Ltau,
                  ptr(self.part), self.nwg, st)
             call("qf_obs_reduce", ptr(self.part), self.nwg, self.V, ptr(self.S[0]), st)
             if lin:
@@ -127,7 +129,7 @@ class BondBondEstimator:
         first = True
         for p0 in range(0, count, self.chunk):
             n = min(self.chunk, count - p0)
-            call("qf_obs_fields", prec, obs, ptr(eta), ptr(G), ptr(phix), ptr(self.pairs), p0, n, self.nrv, self.L, self.Ltau,
+            call("qf_obs_fields", prec, obs, ptr(eta), ptr(G), ptr(self.U), ptr(self.pairs), p0, n, self.nrv, self.L, self.Ltau,
                  ptr(self.A), ptr(self.B), st)
             ah = self._fft(self.A, n)
             bh = ah if lin else self._fft(self.B, n)

@@ -113,13 +113,13 @@ def test_fused_skips_malformed_pairs(dev):
     N, V = L * L * Lt, L * L
     eta, G, boson = _inputs(L, Lt, nrv, seed=4)
     e_d, G_d, _ = _dev(eta, G, boson, dev, torch.complex128)
-    phix = torch.from_numpy(boson[0, 0].transpose(2, 1, 0).copy()).to(dev)
+    U = torch.exp(1j * torch.from_numpy(boson[0, 0].transpose(2, 1, 0).copy())).to(dev)
     nwg = _lib.lib().qf_obs_fused_grid(_lib.QF_PREC_F64, L)
     part = torch.empty(2 * nwg, V, dtype=torch.complex128, device=dev)
     out = torch.empty(2, V, dtype=torch.complex128, device=dev)
     for i, pl in enumerate(([0, 1], [0, 1, 1, nrv, -1, 2])):
         pairs = torch.tensor(pl, dtype=torch.int32, device=dev)
-        call("qf_obs_fused", _lib.QF_PREC_F64, _lib.QF_OBS_T1, ptr(e_d), ptr(G_d), ptr(phix), ptr(pairs), len(pl) // 2, nrv, L, Lt,
+        call("qf_obs_fused", _lib.QF_PREC_F64, _lib.QF_OBS_T1, ptr(e_d), ptr(G_d), ptr(U), ptr(pairs), len(pl) // 2, nrv, L, Lt,
              ptr(part), nwg, stream_handle())
         call("qf_obs_reduce", ptr(part), nwg, V, ptr(out[i]), stream_handle())
     torch.cuda.synchronize()
@@ -133,11 +133,11 @@ def test_malformed_pair_list_gives_zero_fields(dev):
     N = L * L * Lt
     eta, G, boson = _inputs(L, Lt, nrv, seed=2)
     e_d, G_d, _ = _dev(eta, G, boson, dev, torch.complex128)
-    phix = torch.zeros(N, dtype=torch.float64, device=dev)
+    U = torch.ones(N, dtype=torch.complex128, device=dev)
     pairs = torch.tensor([0, 1, 1, nrv, -1, 2], dtype=torch.int32, device=dev)   # pair 1 and 2 out of range
     A = torch.full((3, N), 7.0, dtype=torch.complex128, device=dev)
     B = torch.full_like(A, 7.0)
-    call("qf_obs_fields", _lib.QF_PREC_F64, _lib.QF_OBS_T1, ptr(e_d), ptr(G_d), ptr(phix), ptr(pairs), 0, 3, nrv, L, Lt,
+    call("qf_obs_fields", _lib.QF_PREC_F64, _lib.QF_OBS_T1, ptr(e_d), ptr(G_d), ptr(U), ptr(pairs), 0, 3, nrv, L, Lt,
          ptr(A), ptr(B), stream_handle())
     torch.cuda.synchronize()
     assert torch.count_nonzero(A[0]) > 0
