@@ -77,8 +77,10 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=0, help="oracle PCG iterations for cpu_baseline (0 = auto, ~15 s)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="use the built-in tilings instead of the autotuner")
-    ap.add_argument("--probe", choices=["per_chain", "shared"], default=None,
-                    help="only launch a calibration copy and the roofline M^dag M three times each (for rocprofv3 --pmc)")
+    ap.add_argument("--probe", choices=["per_chain", "shared", "bond_bond"], default=None,
+                    help="per_chain / shared: only launch a calibration copy and the roofline M^dag M three times each "
+                         "(for rocprofv3 --pmc); bond_bond: only the bond-bond estimators of one chain at L=36, "
+                         "Ltau=360, Nrv=40, twice (for rocprofv3)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batching (L=48, 128 Z2 RHS; ragged set) and L=66 trajectory measurements")
     ap.add_argument("--seed", type=int, default=2024)
@@ -249,6 +251,24 @@ def mdagm_per_chain(torch, K, L, Lt, nchain, reps, dev, tune=True):
             "tiling": K.get_tiling(0, nchain, L, Lt)}
 
 
+def probe_bond_bond():
+    """The bond-bond estimators of one chain (L=36, Ltau=360, Nrv=40, complex64, synthetic
+    G_eta), evaluated twice; prints the time of the second evaluation."""
+    import torch
+    from qed_fermion_amd.observables import BondBondEstimator
+    L, Lt, nrv, dev = 36, 360, 40, torch.device("cuda:0")
+    N = L * L * Lt
+    g = torch.Generator(device="cpu").manual_seed(1)
+    z2 = lambda: (torch.randint(0, 2, (nrv, N), generator=g) * 2 - 1).to(torch.float32)
+    eta = (torch.complex(z2(), z2()) / math.sqrt(2)).to(dev)
+    G = torch.randn(nrv, N, generator=g, dtype=torch.complex64).to(dev)
+    bos = ((torch.rand(1, 2, L, L, Lt, generator=g, dtype=torch.float64) * 2 - 1) * math.pi).to(dev, torch.float32)
+    est = BondBondEstimator(L, Lt, nrv, device=dev)
+    t = timed(torch, lambda: est.bond_bond(eta, G, bos))
+    print(json.dumps({"probe": "bond_bond", "L": L, "Ltau": Lt, "nrv": nrv, "method": est.method, "nwg": est.nwg,
+                      "ms_per_chain": t * 1e3}), flush=True)
+
+
 def probe(args):
     """Three launches of a plain copy of known size, then three of the roofline
     M^dag M (per-chain fields: --roof-chains chains; shared: --roof-nrhs RHS on
@@ -390,6 +410,9 @@ def main():
                                    "noise vectors would shard across GPUs without communication, but no multi-GPU "
                                    "path is built"}))
         sys.exit(2)
+    if args.probe == "bond_bond":
+        probe_bond_bond()
+        return
     if args.probe:
         probe(args)
         return
