@@ -190,13 +190,17 @@ def extras(torch, K, args, dev):
     N = L * L * Lt
     rng = np.random.default_rng(66)
     ph66 = torch.from_numpy(rng.uniform(-np.pi, np.pi, (1, 2 * N))).to(dev)
+    tune66 = None
+    if not args.no_tune:   # one chain of a large lattice: the tau-run length decides how many CUs get work
+        tune66 = autotune.tune(0, L, Lt, 1, kind="pcg", persist=False)
+        autotune.tune(1, L, Lt, 1, kind="pcg", persist=False)
     eng = TrajectoryEngine(L, Lt, 1, precision="mixed", ru_period=args.ru_period, dtau=0.1, J=1.0, K=1.0, compact=False,
                            m=1.0, delta_t=0.06, n_leap=20, max_iter=args.ref_max_iter, rtol=args.ref_rtol, device=dev)
     eng.set_phi(ph66)
     eng.capture(n_warmup=1)
     t66 = timed(torch, eng.replay)
     out["trajectory_L66_20step_mixed"] = {"s_per_trajectory": t66, "solves": 21, "max_iter": args.ref_max_iter,
-                                          "max_residual": eng.err.max().item()}
+                                          "max_residual": eng.err.max().item(), "tiling": tune66}
     log(f"[bench] L=66 20-step mixed trajectory: {t66:.3f} s")
     del eng
     return out
