@@ -213,7 +213,7 @@ int qf_obs_diag(int prec, const void* eta, const void* G, const void* phix, int 
  * of Ahat), reduced with qf_obs_reduce.  count: pairs (QF_OBS_V: noise
  * vectors, <= nrv).  Returns QF_ERR_UNSUPPORTED where the slices do not fit
  * in LDS (complex128 with L > 58, any L > 78). */
-int qf_obs_fused_grid(int prec, int L);   /* work-groups that fill the device; 0 = unsupported */
+int qf_obs_fused_grid(int prec, int L);   /* work-groups that fill the device with the pair estimators; 0 = unsupported */
 int qf_obs_fused(int prec, int obs, const void* eta, const void* G, const void* U, const int* pairs, int count, int nrv, int L, int Ltau,
                  void* part, int nwg, void* stream);
 

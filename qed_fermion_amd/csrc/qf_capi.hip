@@ -575,15 +575,22 @@ template <typename R, int O, typename F> int obs_fused_pick(int L, F&& f) {
     const bool dual = O != OBS_V && obs_fused_lds<R>(L, O, true) <= kLds;
     const size_t lds = obs_fused_lds<R>(L, O, dual);
     if (lds > kLds || V > 6 * 1024) return fail(QF_ERR_UNSUPPORTED, "fused estimator kernel: %zu bytes of LDS at L = %d", lds, L);
-    auto by_k = [&](auto dual_tag) -> int {
-        constexpr bool D = decltype(dual_tag)::value;
-        if (V <= 1024) return f(obs_fused_kernel<R, O, kObsThreads, 2, D>, lds);
-        if (V <= 2048) return f(obs_fused_kernel<R, O, kObsThreads, 4, D>, lds);
-        if (V <= 4096) return f(obs_fused_kernel<R, O, kObsThreads, 8, D>, lds);
-        return f(obs_fused_kernel<R, O, kObsThreads, 12, D>, lds);
+    int m = L;                      // only radices 2, 3, 4 (L = 2^a 3^b)?
+    while (m % 2 == 0) m /= 2;
+    while (m % 3 == 0) m /= 3;
+    const bool small = m == 1;
+    auto by_k = [&](auto dual_tag, auto small_tag) -> int {
+        constexpr bool D = decltype(dual_tag)::value, SM = decltype(small_tag)::value;
+        if (V <= 1024) return f(obs_fused_kernel<R, O, kObsThreads, 2, D, SM>, lds);
+        if (V <= 2048) return f(obs_fused_kernel<R, O, kObsThreads, 4, D, SM>, lds);
+        if (V <= 4096) return f(obs_fused_kernel<R, O, kObsThreads, 8, D, SM>, lds);
+        return f(obs_fused_kernel<R, O, kObsThreads, 12, D, SM>, lds);
     };
-    if constexpr (O == OBS_V) return by_k(std::false_type{});
-    else return dual ? by_k(std::true_type{}) : by_k(std::false_type{});
+    auto by_small = [&](auto dual_tag) -> int {
+        return small ? by_k(dual_tag, std::true_type{}) : by_k(dual_tag, std::false_type{});
+    };
+    if constexpr (O == OBS_V) return by_small(std::false_type{});
+    else return dual ? by_small(std::true_type{}) : by_small(std::false_type{});
 }
 
 template <typename R, typename F> int obs_fused_dispatch(int obs, int L, F&& f) {
@@ -1220,11 +1227,12 @@ int qf_obs_fused_grid(int prec, int L) {
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1) return 0;
-    // resident work-groups per CU of the least-occupying estimator kernel at this L
+    // resident work-groups per CU of the least-occupying pair-estimator kernel at this L (Tvv's
+    // kernel, a twentieth of the work at Nrv = 40, may then run its work-groups in two rounds)
     int best = 1 << 30;
     auto occ = [&](auto tag) {
         using R = decltype(tag);
-        for (int obs = QF_OBS_T1; obs <= QF_OBS_V; ++obs) {
+        for (int obs = QF_OBS_T1; obs <= QF_OBS_T4; ++obs) {
             const int rc = obs_fused_dispatch<R>(obs, L, [&](auto kernel, size_t lds) -> int {
                 int b = 0;
                 if (set_lds(kernel, lds) != QF_OK || hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, kObsThreads, lds) != hipSuccess)

@@ -360,7 +360,9 @@ __device__ __forceinline__ void stockham_stage_any(const cplx<R>* src, cplx<R>* 
 // Forward 2-D transforms of the n arrays [n][L][L] at X (LDS) in place, T
 // ([n][L][L]) as the ping-pong buffer: rows of all arrays, then columns;
 // 2 * nfac stages leave the result in X.
-template <typename R>
+// SMALL: every radix is 2, 3 or 4 (L = 2^a 3^b), so the other butterflies are not compiled
+// in and do not hold registers.
+template <typename R, bool SMALL>
 __device__ void stockham_2d(cplx<R>* X, cplx<R>* T, const cplx<R>* tw, int L, int n, unsigned long long facs, int nfac) {
     cplx<R>* src = X;
     cplx<R>* dst = T;
@@ -369,13 +371,19 @@ __device__ void stockham_2d(cplx<R>* X, cplx<R>* T, const cplx<R>* tw, int L, in
         int Ns = 1;
         for (int f = 0; f < nfac; ++f) {
             const int P = (int)((facs >> (8 * f)) & 0xFF);
-            switch (P) {
-                case 2: stockham_stage<R, 2>(src, dst, tw, L, Ns, n * L, col); break;
-                case 3: stockham_stage<R, 3>(src, dst, tw, L, Ns, n * L, col); break;
-                case 4: stockham_stage<R, 4>(src, dst, tw, L, Ns, n * L, col); break;
-                case 5: stockham_stage<R, 5>(src, dst, tw, L, Ns, n * L, col); break;
-                case 7: stockham_stage<R, 7>(src, dst, tw, L, Ns, n * L, col); break;
-                default: stockham_stage_any<R>(src, dst, tw, L, P, Ns, n * L, col); break;
+            if constexpr (SMALL) {
+                if (P == 4) stockham_stage<R, 4>(src, dst, tw, L, Ns, n * L, col);
+                else if (P == 3) stockham_stage<R, 3>(src, dst, tw, L, Ns, n * L, col);
+                else stockham_stage<R, 2>(src, dst, tw, L, Ns, n * L, col);
+            } else {
+                switch (P) {
+                    case 2: stockham_stage<R, 2>(src, dst, tw, L, Ns, n * L, col); break;
+                    case 3: stockham_stage<R, 3>(src, dst, tw, L, Ns, n * L, col); break;
+                    case 4: stockham_stage<R, 4>(src, dst, tw, L, Ns, n * L, col); break;
+                    case 5: stockham_stage<R, 5>(src, dst, tw, L, Ns, n * L, col); break;
+                    case 7: stockham_stage<R, 7>(src, dst, tw, L, Ns, n * L, col); break;
+                    default: stockham_stage_any<R>(src, dst, tw, L, P, Ns, n * L, col); break;
+                }
             }
             __syncthreads();
             Ns *= P;
@@ -390,7 +398,7 @@ __device__ void stockham_2d(cplx<R>* X, cplx<R>* T, const cplx<R>* tw, int L, in
 // (Bhat = Ahat) also part[gridDim.x + w][k] = sum Ahat(k).  NT threads,
 // MAXK * NT >= L*L.  DUAL: A and B are transformed together as one batch of
 // 2L lines (LDS [A | B] + [T | T]); otherwise one after the other through T.
-template <typename R, int OBS, int NT, int MAXK, bool DUAL>
+template <typename R, int OBS, int NT, int MAXK, bool DUAL, bool SMALL>
 __global__ void __launch_bounds__(NT) obs_fused_kernel(ObsFusedArgs<R> a) {
     extern __shared__ __align__(16) unsigned char obs_lds[];
     const int L = a.L, V = L * L;
@@ -434,12 +442,12 @@ __global__ void __launch_bounds__(NT) obs_fused_kernel(ObsFusedArgs<R> a) {
         }
         __syncthreads();
         if constexpr (OBS == OBS_V) {
-            stockham_2d<R>(A, T, tw, L, 1, a.facs, a.nfac);
+            stockham_2d<R, SMALL>(A, T, tw, L, 1, a.facs, a.nfac);
         } else if constexpr (DUAL) {
-            stockham_2d<R>(A, T, tw, L, 2, a.facs, a.nfac);
+            stockham_2d<R, SMALL>(A, T, tw, L, 2, a.facs, a.nfac);
         } else {
-            stockham_2d<R>(A, T, tw, L, 1, a.facs, a.nfac);
-            stockham_2d<R>(B, T, tw, L, 1, a.facs, a.nfac);
+            stockham_2d<R, SMALL>(A, T, tw, L, 1, a.facs, a.nfac);
+            stockham_2d<R, SMALL>(B, T, tw, L, 1, a.facs, a.nfac);
         }
         const cplx<R>* Bh = (OBS == OBS_V) ? A : B;
 #pragma unroll
