@@ -204,8 +204,8 @@ int qf_obs_reduce(const void* part, int nw, int n, void* out, void* stream);
  * vectors and sites, as part[block][3] double2 (nblk blocks; reduce with
  * qf_obs_reduce(part, nblk, 3, ...)): 0: G(r+x) e(r-x) U(r) U(r-x) (T1),
  * 1: G(r) e(r) (T21, T2), 2: G(r) e(r+2x) conj(U(r) U(r+x)) (T4); e = conj(eta),
- * U = exp(i phix). */
-int qf_obs_diag(int prec, const void* eta, const void* G, const void* phix, int nrv, int L, int Ltau, void* part, int nblk, void* stream);
+ * U = exp(i phix) from qf_obs_phase. */
+int qf_obs_diag(int prec, const void* eta, const void* G, const void* U, int nrv, int L, int Ltau, void* part, int nblk, void* stream);
 /* The whole pair sum of one estimator in one launch, no FFT library: each
  * work-group builds a and b of one (tau slice, pair) in LDS, transforms both
  * with its own mixed-radix FFT and adds Ahat(k) Bhat(-k) in FP64 registers.

@@ -1196,23 +1196,23 @@ int qf_obs_kspace(int prec, const void* A_hat, const void* B_hat, long long nsli
 int qf_obs_reduce(const void* part, int nw, int n, void* out, void* stream) {
     if (!part || !out) return fail(QF_ERR_INVALID, "null pointer");
     if (nw < 1 || n < 1) return fail(QF_ERR_INVALID, "bad sizes");
-    hipLaunchKernelGGL(obs_reduce_kernel, dim3((unsigned)std::min(((long long)n + 255) / 256, 4096LL)), dim3(256), 0, (hipStream_t)stream,
-                       (const double2*)part, nw, n, (double2*)out);
+    hipLaunchKernelGGL(obs_reduce_kernel, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, (hipStream_t)stream, (const double2*)part, nw, n,
+                       (double2*)out);
     QF_LAUNCHED();
     return QF_OK;
 }
 
-int qf_obs_diag(int prec, const void* eta, const void* G, const void* phix, int nrv, int L, int Ltau, void* part, int nblk, void* stream) {
-    if (!eta || !G || !phix || !part) return fail(QF_ERR_INVALID, "null pointer");
+int qf_obs_diag(int prec, const void* eta, const void* G, const void* U, int nrv, int L, int Ltau, void* part, int nblk, void* stream) {
+    if (!eta || !G || !U || !part) return fail(QF_ERR_INVALID, "null pointer");
     if (L < 2 || Ltau < 1 || nrv < 1 || nblk < 1 || nblk > 65535) return fail(QF_ERR_INVALID, "bad sizes");
     const long long N = (long long)L * L * Ltau;
     hipStream_t s = (hipStream_t)stream;
     if (prec == QF_PREC_F32)
-        hipLaunchKernelGGL((obs_diag_kernel<float>), dim3(nblk), dim3(256), 0, s, (const float2*)eta, (const float2*)G, (const float*)phix, nrv,
+        hipLaunchKernelGGL((obs_diag_kernel<float>), dim3(nblk), dim3(256), 0, s, (const float2*)eta, (const float2*)G, (const float2*)U, nrv,
                            L, N, (double2*)part);
     else if (prec == QF_PREC_F64)
         hipLaunchKernelGGL((obs_diag_kernel<double>), dim3(nblk), dim3(256), 0, s, (const double2*)eta, (const double2*)G,
-                           (const double*)phix, nrv, L, N, (double2*)part);
+                           (const double2*)U, nrv, L, N, (double2*)part);
     else
         return fail(QF_ERR_INVALID, "prec %d", prec);
     QF_LAUNCHED();

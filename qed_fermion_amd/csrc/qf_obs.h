@@ -161,16 +161,30 @@ __global__ void __launch_bounds__(256) obs_kspace_kernel(const cplx<R>* Ah, cons
     }
 }
 
-// out[k] = sum_w part[w * n + k], w in order
+// out[k] = sum_w part[w * n + k] in a fixed order: a block of 256 threads takes
+// 16 consecutive k, and 16 lanes per k each sum every 16th row (w = lane, lane
+// + 16, ...) before the 16 partial sums are added in lane order.
 __global__ void __launch_bounds__(256) obs_reduce_kernel(const double2* part, int nw, int n, double2* out) {
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        double r = 0, i = 0;
-        for (int w = 0; w < nw; ++w) {
+    __shared__ double2 red[16][17];
+    const int kk = threadIdx.x & 15, lane = threadIdx.x >> 4;
+    const int k = blockIdx.x * 16 + kk;
+    double r = 0, i = 0;
+    if (k < n) {
+        for (int w = lane; w < nw; w += 16) {
             const double2 p = part[(long long)w * n + k];
             r += p.x;
             i += p.y;
         }
-        out[k] = make_double2(r, i);
+    }
+    red[lane][kk] = make_double2(r, i);
+    __syncthreads();
+    if (lane == 0 && k < n) {
+        double sr = 0, si = 0;
+        for (int j = 0; j < 16; ++j) {
+            sr += red[j][kk].x;
+            si += red[j][kk].y;
+        }
+        out[k] = make_double2(sr, si);
     }
 }
 
@@ -180,7 +194,7 @@ __global__ void __launch_bounds__(256) obs_reduce_kernel(const double2* part, in
 //   1: G(r) e(r)                        T21, T2 at (0, 0) (1496-1500, 1553-1557)
 //   2: G(r) e(r++) Ub(r) Ub(r+)         T4 at (0, 1)     (1601-1608)
 template <typename R>
-__global__ void __launch_bounds__(256) obs_diag_kernel(const cplx<R>* eta, const cplx<R>* G, const R* phix, int nrv, int L,
+__global__ void __launch_bounds__(256) obs_diag_kernel(const cplx<R>* eta, const cplx<R>* G, const cplx<R>* Uf, int nrv, int L,
                                                        long long N, double2* part) {
     __shared__ double red[6][256];
     const long long total = (long long)nrv * N;
@@ -194,7 +208,7 @@ __global__ void __launch_bounds__(256) obs_diag_kernel(const cplx<R>* eta, const
         const long long xp = row + (x + 1) % L, xm = row + (x + L - 1) % L, xpp = row + (x + 2) % L;
         const cplx<R>* e = eta + sv * N;
         const cplx<R>* g = G + sv * N;
-        const cplx<R> U0 = expi<R>(phix[site]), Um = expi<R>(phix[xm]), Up = expi<R>(phix[xp]);
+        const cplx<R> U0 = Uf[site], Um = Uf[xm], Up = Uf[xp];
         const cplx<R> d0 = cmul<R>(cmul<R>(g[xp], cconj<R>(e[xm])), cmul<R>(U0, Um));
         const cplx<R> d1 = cmul<R>(g[site], cconj<R>(e[site]));
         const cplx<R> d2 = cmul<R>(cmul<R>(g[site], cconj<R>(e[xpp])), cconj<R>(cmul<R>(U0, Up)));

@@ -145,22 +145,28 @@ class BondBondEstimator:
         return torch.fft.fft2(S.view(self.L, self.L)) / (self.V * self.V)
 
     def _diag(self, eta, G, phix):
-        call("qf_obs_diag", _PREC[self.dtype], ptr(eta), ptr(G), ptr(phix), self.nrv, self.L, self.Ltau, ptr(self.dpart), self.nblk,
+        call("qf_obs_diag", _PREC[self.dtype], ptr(eta), ptr(G), ptr(self.U), self.nrv, self.L, self.Ltau, ptr(self.dpart), self.nblk,
              stream_handle())
         call("qf_obs_reduce", ptr(self.dpart), self.nblk, 3, ptr(self.dsum), stream_handle())
         return self.dsum / (self.nrv * self.N)
 
     # ------------------------------------------------------------ estimators
-    def T(self, name, eta, G_eta, boson):
-        """T1 / T21 / T2 / T4 (stochastic_estimator.py:1393-1610): [Ly, Lx] in eta's dtype."""
+    def T(self, name, eta, G_eta, boson, diag=None):
+        """T1 / T21 / T2 / T4 (stochastic_estimator.py:1393-1610): [Ly, Lx] in eta's dtype.
+        diag: the three same-noise-vector sums of these inputs when already computed."""
         obs = _NAMES[name]
         eta, G, phix = self._inputs(eta, G_eta, boson)
         self._kspace_sum(obs, eta, G, phix, self.npairs)
         out = -self._corr(self.S[0]) / (self.npairs * self.Ltau)
         (iy, ix), which = _DIAG[name][0](self.L), _DIAG[name][1]
-        d = self._diag(eta, G, phix)
+        d = self._diag(eta, G, phix) if diag is None else diag
         out[iy, ix] += d[which]
         return out.to(self.dtype)
+
+    def diag_sums(self, eta, G_eta, boson):
+        """The three same-noise-vector sums the pair estimators add at one Delta each."""
+        eta, G, phix = self._inputs(eta, G_eta, boson)
+        return self._diag(eta, G, phix).clone()
 
     def Tvv(self, eta, G_eta, boson):
         """(Tvv, Tv) (stochastic_estimator.py:1612-1654): [Ly, Lx] each, in eta's dtype."""
@@ -174,10 +180,11 @@ class BondBondEstimator:
         """get_bond_bond_per_b2 (2594-2641): {'BB0_r', 'BB_r', 'B_r', 'BB_k'}, real [Ly, Lx].
         runners: optional {name: callable(eta, G_eta, boson)} replacing the direct calls (graph runners)."""
         f = runners or {}
-        t1 = f.get("T1", lambda *a: self.T("T1", *a))(eta, G_eta, boson)
-        t21 = f.get("T21", lambda *a: self.T("T21", *a))(eta, G_eta, boson)
-        t2 = f.get("T2", lambda *a: self.T("T2", *a))(eta, G_eta, boson)
-        t4 = f.get("T4", lambda *a: self.T("T4", *a))(eta, G_eta, boson)
+        d = None if runners else self.diag_sums(eta, G_eta, boson)    # shared by the four pair estimators
+        t1 = f.get("T1", lambda *a: self.T("T1", *a, diag=d))(eta, G_eta, boson)
+        t21 = f.get("T21", lambda *a: self.T("T21", *a, diag=d))(eta, G_eta, boson)
+        t2 = f.get("T2", lambda *a: self.T("T2", *a, diag=d))(eta, G_eta, boson)
+        t4 = f.get("T4", lambda *a: self.T("T4", *a, diag=d))(eta, G_eta, boson)
         tvv, tv = f.get("Tvv", self.Tvv)(eta, G_eta, boson)
         BB_r = (t1 + t21 + t2 + t4) * 2 + tvv * 4
         BB_k = reorder_fft_grid2(torch.fft.ifft2(BB_r, norm="forward"))
