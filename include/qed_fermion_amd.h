@@ -66,6 +66,12 @@ int qf_clear_tuning(void);
 /* out4 = {patch height, tau slices per operator work-group, tau slices per
  * update work-group, operator work-groups per right-hand side}. */
 int qf_get_tiling(int prec, int nrhs, int L, int Ltau, int* out4);
+/* PCG x updates in pairs (on = 1, the default): an iteration that is not
+ * followed by a read of x holds x += alpha p and the next iteration applies
+ * both steps, so x and p are read every other iteration.  on = 0 updates x in
+ * every iteration as hmc_sampler_batch.py:802 does; the results are the same.
+ * Returns the previous setting.  No counterpart in the reference. */
+int qf_set_paired_x_updates(int on);
 
 /* y = op(phi) x for nrhs right-hand sides.  op = QF_OP_M: get_M_sparse's M
  * (hmc_sampler_batch.py:1884-1906); QF_OP_MDAG: M^dag (_C.mh_vec, :2406);

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -66,6 +67,7 @@ struct ShapeTuning {
 };
 std::vector<ShapeTuning> g_shape_tune;
 std::mutex g_tune_mu;
+std::atomic<int> g_paired_x{1};   // qf_set_paired_x_updates
 
 // Work-groups of up to 512 threads get 256 VGPRs per lane; only the 2x2
 // patch may use 1024-thread work-groups (L up to 64).
@@ -368,6 +370,7 @@ struct CgWork {
     double* pap;
     CgState* st;
     double* normb;
+    double* xal;    // alpha of a held iteration, [nrhs]
     double* rp64;   // mixed only
     double* ru_rr;  // mixed only: [2][nrhs]
     double* ru_it;  // mixed only: [2][nrhs]
@@ -391,6 +394,7 @@ CgWork carve(void* base, int prec, int mixed, int nrhs, int L, int Lt, const Til
     w.pap = (double*)take(sizeof(double) * nrhs * t.nA);
     w.st = (CgState*)take(sizeof(CgState) * 2 * nrhs);
     w.normb = (double*)take(sizeof(double) * nrhs);
+    w.xal = (double*)take(sizeof(double) * nrhs);
     w.rp64 = mixed ? (double*)take(sizeof(double) * nrhs * t64.nA) : nullptr;
     w.ru_rr = mixed ? (double*)take(sizeof(double) * 2 * nrhs) : nullptr;
     w.ru_it = mixed ? (double*)take(sizeof(double) * 2 * nrhs) : nullptr;
@@ -431,7 +435,7 @@ int pcg_iterate(const R* phi, long long phi_cs, const cplx<R>* b_in, cplx<R>* x,
 
     // Mixed precision: reliable-update checks every kRuCheck iterations (see
     // ru_decide_flush_kernel); the check after iteration i uses parity c & 1.
-    constexpr int kRuCheck = 5;
+    constexpr int kRuCheck = 10;
     const bool mixed = ru_period > 0;
     const long long n_el = (long long)nrhs * N;
     const int ru_blocks = (int)std::max(1LL, std::min((N + 1023) / 1024, 512LL));
@@ -457,8 +461,16 @@ int pcg_iterate(const R* phi, long long phi_cs, const cplx<R>* b_in, cplx<R>* x,
     };
     (void)n_el;
 
+    // x updates go in pairs (xmode of cg_update_kernel): an iteration holds its
+    // step unless x is read after it (a reliable-update check, the end), and the
+    // iteration after a held one applies both steps.
+    const bool paired = g_paired_x.load() != 0;
+    bool held = false;
     for (int i = 0; i < max_iter; ++i) {
         const int cur = i & 1, prv = cur ^ 1;
+        const bool ru_due = mixed && (i + 1) % kRuCheck == 0 && i + 1 < max_iter;
+        const int xmode = !paired ? 0 : held ? 2 : (ru_due || i + 1 == max_iter ? 0 : 1);
+        held = xmode == 1;
         m.iter = i;
         // mixed precision: a chain may only stop on a true (FP64) residual,
         // i.e. right after a reliable update (stop_flag of the last check);
@@ -484,9 +496,10 @@ int pcg_iterate(const R* phi, long long phi_cs, const cplx<R>* b_in, cplx<R>* x,
         u.pap = w.pap; u.st = w.st + cur * nrhs;
         u.rz_part = w.rzp + cur * ps; u.rr_part = w.rrp + cur * ps;
         u.ru_flag = nullptr;
+        u.xmode = xmode; u.pprev = (const cplx<R>*)w.P[prv]; u.xalpha = w.xal; u.st_held = w.st + prv * nrhs;
         rc = launch_cgu<R, CGU_ITER>(u, t, nrhs, s);
         if (rc) return rc;
-        if (mixed && (i + 1) % kRuCheck == 0 && i + 1 < max_iter) {
+        if (ru_due) {
             // reliable update where needed: x_hi += x_lo; r = b - A64 x_hi (FP64); z = P r
             rc = reliable_update(i + 1, cur, 0, cur);
             if (rc) return rc;
@@ -646,6 +659,8 @@ int qf_clear_tuning(void) {
     g_tune[1] = Tuning();
     return QF_OK;
 }
+
+int qf_set_paired_x_updates(int on) { return g_paired_x.exchange(on ? 1 : 0); }
 
 int qf_get_tiling(int prec, int nrhs, int L, int Ltau, int* out4) {
     int rc = check_lattice(L, Ltau, nrhs);

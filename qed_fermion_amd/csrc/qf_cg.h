@@ -16,6 +16,14 @@
 //   partial sums of Re(r^dag z) and |r|^2 per block (FP64).
 // r is double buffered (rold -> rnew) because z needs the new r of the
 // neighbouring slices, which it recomputes on the fly from rold and Op.
+//
+// x is only needed at the end of a solve and at a reliable update, so the
+// iterations may hold one step back: a held iteration (xmode 1) keeps its
+// alpha and leaves x untouched; the next iteration (xmode 2) applies
+// x += alpha' p' + alpha p with p' still in the other p buffer.  In FP32 or
+// FP64 the stored x equals the step-by-step x bit for bit (the same two
+// rounded multiply-adds, the first no longer stored in between).  A chain that
+// stops right after a held iteration applies alpha' p' alone.
 #pragma once
 #include "qf_reduce.h"
 
@@ -41,6 +49,11 @@ struct CguArgs {
     double* rz_part;       // [nrhs][nblk]
     double* rr_part;
     const int* ru_flag;    // REFRESH: skip chains whose reliable-update flag is 0 (nullptr: never skip)
+    int xmode;             // ITER: 0 x += alpha p; 1 hold (alpha into xalpha, x untouched);
+                           //       2 x += alpha' p' + alpha p (alpha', p' of the held iteration)
+    const cplx<R>* pprev;  // ITER, xmode 2: p' (search direction of the held iteration)
+    double* xalpha;        // ITER: [nrhs] alpha of the held iteration (xmode 1 writes, 2 reads)
+    const CgState* st_held;  // ITER, xmode 2: state written by step A of the held iteration
     const Prob* prob;      // ragged batch (nullptr: uniform; grid (nsx, tiles, nrhs))
     int nprob;
 };
@@ -84,7 +97,21 @@ __global__ void __launch_bounds__(TX * TT / SPT) cg_update_kernel(CguArgs<R> A) 
     const int sc = col ? s : 0;   // loads of padding columns read site 0 (never stored)
 
     if constexpr (MODE == CGU_ITER) {
-        if (A.st[rho].act == 0.0) return;   // converged chain: x and r stay frozen (uniform)
+        if (A.st[rho].act == 0.0) {   // converged chain: x and r stay frozen (uniform)
+            if (A.xmode == 2 && A.st_held[rho].act != 0.0) {   // stopped right after a held iteration
+                const R ah = (R)A.xalpha[rho];
+#pragma unroll
+                for (int j = 0; j < SPT; ++j) {
+                    const int tau = t0 + sr + j * RT;
+                    if (col && tau < Lt) {
+                        const long long io = vo + (long long)tau * Vs + s;
+                        const cplx<R> xv = A.x[io], pv = A.pprev[io];
+                        A.x[io] = mk<R>(xv.x + ah * pv.x, xv.y + ah * pv.y);
+                    }
+                }
+            }
+            return;
+        }
     }
     if constexpr (MODE == CGU_REFRESH) {
         if (A.ru_flag != nullptr && A.ru_flag[rho] == 0) return;
@@ -121,14 +148,17 @@ __global__ void __launch_bounds__(TX * TT / SPT) cg_update_kernel(CguArgs<R> A) 
             if constexpr (MODE == CGU_ITER) ow[j] = op[i];
         }
     }
-    cplx<R> pp[SPT], xx[SPT];
+    cplx<R> pp[SPT], xx[SPT], ph[SPT];
 #pragma unroll
     for (int j = 0; j < SPT; ++j) {
-        pp[j] = xx[j] = mk<R>(0, 0);
+        pp[j] = xx[j] = ph[j] = mk<R>(0, 0);
         if constexpr (MODE == CGU_ITER) {
             const long long io = vo + (long long)min(t0 + sr + j * RT, Lt - 1) * Vs + sc;
-            pp[j] = A.p[io];
-            xx[j] = A.x[io];
+            if (A.xmode != 1) {
+                pp[j] = A.p[io];
+                xx[j] = A.x[io];
+            }
+            if (A.xmode == 2) ph[j] = A.pprev[io];
         }
     }
 
@@ -138,6 +168,12 @@ __global__ void __launch_bounds__(TX * TT / SPT) cg_update_kernel(CguArgs<R> A) 
         const double* src[1] = {A.pap + pin};
         sum_partials<1>(src, nA, sums, red);
         alpha = (R)(A.st[rho].rz / sums[0]);
+        if (A.xmode == 1 && blk == 0 && threadIdx.x == 0) A.xalpha[rho] = (double)alpha;
+        if (A.xmode == 2) {   // x_{i} = x_{i-1} + alpha' p', kept in registers
+            const R ah = (R)A.xalpha[rho];
+#pragma unroll
+            for (int j = 0; j < SPT; ++j) xx[j] = mk<R>(xx[j].x + ah * ph[j].x, xx[j].y + ah * ph[j].y);
+        }
 #pragma unroll
         for (int k = 0; k < (PRE ? NLD : SPT); ++k) rw[k] = mk<R>(rw[k].x - alpha * ow[k].x, rw[k].y - alpha * ow[k].y);
     }
@@ -171,7 +207,9 @@ __global__ void __launch_bounds__(TX * TT / SPT) cg_update_kernel(CguArgs<R> A) 
         }
         if (col && tau < Lt) {
             const long long io = vo + (long long)tau * Vs + s;
-            if constexpr (MODE == CGU_ITER) A.x[io] = mk<R>(xx[j].x + alpha * pp[j].x, xx[j].y + alpha * pp[j].y);
+            if constexpr (MODE == CGU_ITER) {
+                if (A.xmode != 1) A.x[io] = mk<R>(xx[j].x + alpha * pp[j].x, xx[j].y + alpha * pp[j].y);
+            }
             if constexpr (MODE == CGU_INIT) A.x[io] = mk<R>(0, 0);
             if constexpr (MODE == CGU_INIT || MODE == CGU_ITER) A.rnew[io] = r;
             A.z[io] = zz;

@@ -125,6 +125,47 @@ def test_pcg_converged_chain_freezes(dev):
         assert err[i].item() <= 1e-6 and er <= 1e-6
 
 
+@pytest.mark.parametrize("prec,rtol,max_iter", [("f64", 1e-10, 600), ("f64", 0.0, 37), ("f32", 1e-5, 600),
+                                              ("mixed", 1e-10, 600), ("mixed", 1e-10, 37)])
+def test_paired_x_updates_equal_stepwise(dev, prec, rtol, max_iter):
+    """x updates held and applied in pairs (the default) give the x of one update per
+    iteration (qf_set_paired_x_updates(0)) bit for bit, with the same iteration counts:
+    chains that stop right after a held iteration (even counts) or after a full one (odd
+    counts), a solve that ends on a held iteration (max_iter odd), and the mixed solver's
+    reliable updates, which read x every few iterations."""
+    from qed_fermion_amd import _lib
+    k = _k()
+    L, Lt, nrhs = 6, 40, 8
+    print(f"[paired x] {prec} rtol={rtol} max_iter={max_iter}", flush=True)
+    phi = make_phi(L, Lt, "gauss", seed=41, nchain=nrhs)   # one field per chain: different stopping iterations
+    R = make_vec(L, Lt, seed=42, nrhs=nrhs)
+    tab = O.precon_table(Lt)
+    rd, cd = (torch.float32, torch.complex64) if prec == "f32" else (torch.float64, torch.complex128)
+    phd = _t(phi.reshape(nrhs, -1), dev, rd)
+    bd = _t(np.stack([O.apply_Mdag(phi[i], R[i], L, Lt) for i in range(nrhs)]), dev, cd)
+    tbd = _t(tab, dev, rd)
+    out = {}
+    for on in (1, 0):
+        prev = _lib.lib().qf_set_paired_x_updates(on)
+        try:
+            if prec == "mixed":
+                x, it, err = k.pcg_mixed_solve(phd, bd, L, Lt, ptab=tbd, rtol=rtol, max_iter=max_iter, ru_period=50)
+            else:
+                x, it, err = k.pcg_solve(phd, bd, L, Lt, ptab=tbd, rtol=rtol, max_iter=max_iter)
+            torch.cuda.synchronize()
+            out[on] = (x.clone(), it.clone(), err.clone())
+        finally:
+            _lib.lib().qf_set_paired_x_updates(prev)
+    assert _lib.lib().qf_set_paired_x_updates(1) == 1   # the default is restored
+    assert torch.equal(out[1][1], out[0][1])
+    assert torch.equal(out[1][2], out[0][2])
+    assert torch.equal(out[1][0], out[0][0]), (out[1][0] - out[0][0]).abs().max().item()
+    its = [int(v) for v in out[1][1].cpu()]
+    print(f"[paired x] iterations {its}", flush=True)
+    if prec != "mixed" and rtol > 0:
+        assert max(its) < max_iter - 1 and {v % 2 for v in its} == {0, 1}, its
+
+
 @pytest.mark.parametrize("L,Lt,kind,rtol,max_iter", [(6, 60, "uniform", 1e-10, 1200), (8, 40, "gauss", 1e-10, 600),
                                                      (4, 30, "gauss", 1e-13, 200)])
 def test_pcg_mixed(dev, L, Lt, kind, rtol, max_iter):
