@@ -255,6 +255,30 @@ def mdagm_per_chain(torch, K, L, Lt, nchain, reps, dev, tune=True):
             "tiling": K.get_tiling(0, nchain, L, Lt)}
 
 
+def stream_copy_rate(torch, K, dev, reps):
+    """Achievable streaming rate: qf_stream_copy of 512 MiB (twice the 256 MB Infinity Cache),
+    one 16-byte element per thread, timed like the operator with HIP events on the launch
+    stream.  Bytes counted: read + write."""
+    nbytes = 512 << 20
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).uniform_()
+    b = torch.empty_like(a)
+    for _ in range(3):
+        K.stream_copy(a, b)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        K.stream_copy(a, b)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    if not torch.equal(a, b):
+        raise RuntimeError("stream copy produced a wrong result")
+    return {"kernel": "stream_copy_kernel (one 16-byte element per thread)", "bytes_each_way": nbytes,
+            "ms_per_launch": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}
+
+
 def probe_bond_bond():
     """The bond-bond estimators of one chain (L=36, Ltau=360, Nrv=40, complex64, synthetic
     G_eta), evaluated twice; prints the time of the second evaluation."""
@@ -472,6 +496,8 @@ def main():
 
     per_chain = mdagm_per_chain(torch, K, L, Lt, args.roof_chains, args.roof_reps, dev, tune=not args.no_tune)
     log(f"[bench] M^dag M, {args.roof_chains} chains with their own fields: {per_chain['GBps']:.0f} GB/s")
+    copy = stream_copy_rate(torch, K, dev, args.roof_reps)
+    log(f"[bench] streaming copy reference: {copy['GBps']:.0f} GB/s")
     extra = None if args.no_extras else extras(torch, K, args, dev)
 
     # ---------------------------------------------------------- CPU baseline (oracle)
@@ -534,7 +560,8 @@ def main():
                      "frac": per_chain["GBps"] / HBM_PEAK_GBS, "traffic": traffic_of(L, Lt, per_chain["nchains"], "per_chain"),
                      "kernel": per_chain["kernel"], "nchains": per_chain["nchains"],
                      "bytes_per_launch": per_chain["bytes_per_launch"], "ms_per_launch": per_chain["ms_per_launch"],
-                     "tiling": per_chain["tiling"]},
+                     "tiling": per_chain["tiling"], "frac_of_stream_copy": per_chain["GBps"] / copy["GBps"]},
+        "stream_copy": copy,
         "mdagm_shared_field": {"GBps": roof["GBps"], "frac": roof["GBps"] / HBM_PEAK_GBS, "kernel": roof["kernel"],
                                "nrhs": roof["nrhs"], "bytes_per_launch": roof["bytes_per_launch"],
                                "ms_per_launch": roof["ms_per_launch"], "traffic": traffic_of(L, Lt, roof["nrhs"], "shared")},

@@ -618,9 +618,29 @@ template <typename R, typename F> int obs_fused_dispatch(int obs, int L, F&& f) 
 
 }  // namespace
 
+// Streaming copy, one 16-byte element per thread (the grid covers the buffer):
+// the achievable-bandwidth reference the operator kernels are compared with.
+__global__ void __launch_bounds__(256) stream_copy_kernel(const float4* __restrict__ a, float4* __restrict__ b, long long n) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
 extern "C" {
 
 int qf_version(void) { return QF_ABI_VERSION; }
+
+int qf_stream_copy(const void* src, void* dst, long long bytes, void* stream) {
+    if (!src || !dst) return fail(QF_ERR_INVALID, "null pointer");
+    if (bytes < 0 || bytes % 16 != 0 || ((uintptr_t)src | (uintptr_t)dst) % 16 != 0)
+        return fail(QF_ERR_INVALID, "copy of %lld bytes: size and addresses must be multiples of 16", bytes);
+    const long long n = bytes / 16;
+    if ((n + 255) / 256 > 0x7fffffffLL) return fail(QF_ERR_INVALID, "copy of %lld bytes is too large", bytes);
+    if (n == 0) return QF_OK;
+    hipLaunchKernelGGL(stream_copy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const float4*)src, (float4*)dst, n);
+    QF_LAUNCHED();
+    return QF_OK;
+}
 
 const char* qf_last_error(void) { return g_err.c_str(); }
 

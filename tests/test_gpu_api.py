@@ -251,3 +251,17 @@ def test_autograd_force(dev):
     ph2 = ph.detach().clone().requires_grad_(True)
     fermion_action(ph2, ps, L, Lt, ptab=ptab, rtol=1e-14, max_iter=3000, recompute=False).sum().backward()
     assert torch.equal(ph2.grad, ph.grad)
+
+
+def test_stream_copy(dev):
+    """The streaming-bandwidth reference copies exactly and rejects sizes that are not whole
+    16-byte elements."""
+    from qed_fermion_amd import kernels as K
+    a = torch.randn(4100, device=dev)   # 16400 bytes, 1025 elements: a partial last work-group
+    b = torch.zeros_like(a)
+    K.stream_copy(a, b)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    c, d = torch.zeros(3, device=dev), torch.zeros(3, device=dev)
+    with pytest.raises(K.QfError):
+        K.stream_copy(c, d)
