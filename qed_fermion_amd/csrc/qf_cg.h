@@ -71,8 +71,13 @@ __global__ void __launch_bounds__(TX * TT / SPT) cg_update_kernel(CguArgs<R> A) 
     constexpr int RT = TT / SPT;      // thread rows
     constexpr int NLD = (ROWS + RT - 1) / RT;
     static_assert(TT % SPT == 0, "slices per thread must divide the tile");
+    // preconditioner rows of the tile's TT slices, padded to 16-byte multiples
+    constexpr int CST = (13 * sizeof(R) + 15) / 16 * 16 / sizeof(R);
     __shared__ cplx<R> ring[PRE ? ROWS * TX : 1];
+    __shared__ __align__(16) R crow[PRE ? TT * CST : 1];
     __shared__ double red[64];
+    constexpr int NTHR = TX * TT / SPT;
+    constexpr int NCE = PRE ? (TT * 13 + NTHR - 1) / NTHR : 1;   // coefficients per thread
     int rho, Vs, Lt, bx, by, blk, nA;
     long long vo, pout, pin;
     const R* ptab = A.ptab;
@@ -123,7 +128,12 @@ __global__ void __launch_bounds__(TX * TT / SPT) cg_update_kernel(CguArgs<R> A) 
         }
     }
 
-    // ---- loads (none depends on the step size)
+    // ---- loads (none depends on the step size); the partials of Re(p^dag Op) first
+    Partials<1, 2> pf{};
+    if constexpr (MODE == CGU_ITER) {
+        const double* src[1] = {A.pap + pin};
+        pf = load_partials<1, 2>(src, nA);
+    }
     const cplx<R>* rold = A.rold + vo;
     const cplx<R>* op = A.op + vo;
     auto slice_of = [&](int h) { return ((t0 - H / 2 + h) % Lt + Lt) % Lt; };
@@ -148,6 +158,17 @@ __global__ void __launch_bounds__(TX * TT / SPT) cg_update_kernel(CguArgs<R> A) 
             if constexpr (MODE == CGU_ITER) ow[j] = op[i];
         }
     }
+    // the stencil's coefficients go through LDS with the ring, so no global load
+    // waits behind the barrier; they are loaded here with the vectors
+    R cv[NCE];
+    if constexpr (PRE) {
+#pragma unroll
+        for (int q = 0; q < NCE; ++q) {
+            const int e = threadIdx.x + q * NTHR;
+            const int row = e / 13, k = e - row * 13;
+            cv[q] = e < TT * 13 ? ptab[min(t0 + row, Lt - 1) * 13 + k] : R(0);
+        }
+    }
     cplx<R> pp[SPT], xx[SPT], ph[SPT];
 #pragma unroll
     for (int j = 0; j < SPT; ++j) {
@@ -166,7 +187,7 @@ __global__ void __launch_bounds__(TX * TT / SPT) cg_update_kernel(CguArgs<R> A) 
     if constexpr (MODE == CGU_ITER) {
         double sums[1];
         const double* src[1] = {A.pap + pin};
-        sum_partials<1>(src, nA, sums, red);
+        sum_partials_pf<1, 2>(pf, src, nA, sums, red);
         alpha = (R)(A.st[rho].rz / sums[0]);
         if (A.xmode == 1 && blk == 0 && threadIdx.x == 0) A.xalpha[rho] = (double)alpha;
         if (A.xmode == 2) {   // x_{i} = x_{i-1} + alpha' p', kept in registers
@@ -183,6 +204,12 @@ __global__ void __launch_bounds__(TX * TT / SPT) cg_update_kernel(CguArgs<R> A) 
             const int h = sr + k * RT;
             if (h < ROWS) ring[h * TX + sx] = rw[k];
         }
+#pragma unroll
+        for (int q = 0; q < NCE; ++q) {
+            const int e = threadIdx.x + q * NTHR;
+            const int row = e / 13;
+            if (e < TT * 13) crow[row * CST + (e - row * 13)] = cv[q];
+        }
         __syncthreads();
     }
 
@@ -194,7 +221,7 @@ __global__ void __launch_bounds__(TX * TT / SPT) cg_update_kernel(CguArgs<R> A) 
         cplx<R> r, zz;
         if constexpr (PRE) {
             r = ring[(st + H / 2) * TX + sx];
-            const R* c = ptab + min(tau, Lt - 1) * 13;
+            const R* c = crow + st * CST;
             zz = mk<R>(0, 0);
 #pragma unroll
             for (int k = 0; k < 13; ++k) {

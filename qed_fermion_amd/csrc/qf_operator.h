@@ -102,6 +102,10 @@ __device__ __forceinline__ void touch_lines(unsigned (&d)[NTOUCH], __amdgpu_buff
     }
 }
 
+// Partial sums of the previous update step that every PCG step-A work-group
+// prefetches per lane of its first wave (up to 256 partials in one round trip).
+constexpr int kMdmPartialPrefetch = 4;
+
 // PIPE = false: one vector per exchange (14 barriers per slice) and only two
 // coefficient sets live, for patches whose pipelined form exceeds the
 // register file (2 x 6 patches, complex128):
@@ -170,7 +174,13 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
         }
     };
 
-    // prologue loads first: they do not depend on the CG scalars
+    // prologue loads first: they do not depend on the CG scalars (the partial sums
+    // those come from are requested before them)
+    Partials<2, kMdmPartialPrefetch> pf{};
+    if constexpr (MODE == MDM_CG) {
+        const double* src[2] = {A.rz_part + pb, A.rr_part + pb};
+        pf = load_partials<2, kMdmPartialPrefetch>(src, nBk);
+    }
     const int tm = (t0 + Lt - 1) % Lt;
     const int t0n = (t0 + 1 == Lt) ? 0 : t0 + 1;
     CoefC<R, BY> c0, c1, c2;
@@ -186,7 +196,7 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
         // beta = rz_new / rz_old).
         double sums[2];
         const double* src[2] = {A.rz_part + pb, A.rr_part + pb};
-        sum_partials<2>(src, nBk, sums, red);
+        sum_partials_pf<2, kMdmPartialPrefetch>(pf, src, nBk, sums, red);
         const CgState St = A.st_prev[rho];
         const double nb = (A.iter == 0) ? sqrt(sums[1]) : A.normb[rho];
         double act, err, rzv, it = St.iters, bt = 0.0;

@@ -55,4 +55,47 @@ __device__ __forceinline__ void sum_partials(const double* const (&src)[NV], int
     __syncthreads();
 }
 
+// sum_partials split in two so that the loads can be issued early: the first
+// NP loads of each lane of the first wave are made by load_partials (before
+// the caller's other loads), the rest and the sum by sum_partials_pf.  Same
+// summation order, hence the same result, as sum_partials.
+template <int NV, int NP>
+struct Partials { double v[NV][NP]; };
+
+template <int NV, int NP>
+__device__ __forceinline__ Partials<NV, NP> load_partials(const double* const (&src)[NV], int n) {
+    const int lane = threadIdx.x & 63;
+    Partials<NV, NP> p;
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            const int i = lane + 64 * j;
+            p.v[k][j] = (threadIdx.x < 64 && i < n) ? src[k][i] : 0.0;
+        }
+    return p;
+}
+
+template <int NV, int NP>
+__device__ __forceinline__ void sum_partials_pf(const Partials<NV, NP>& p, const double* const (&src)[NV], int n,
+                                                double (&out)[NV], double* red) {
+    const int lane = threadIdx.x & 63;
+    if (threadIdx.x < 64) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < NP; ++j)
+                if (lane + 64 * j < n) s += p.v[k][j];
+            for (int i = lane + 64 * NP; i < n; i += 64) s += src[k][i];
+            s = wave_sum(s);
+            if (lane == 0) red[k] = s;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[k] = red[k];
+    __syncthreads();
+}
+
 }  // namespace qf
