@@ -9,7 +9,9 @@
 //
 // A block owns a tile of sites x tau slices; the new residual of the tile and
 // of its 12-slice halo is staged in LDS, so every r value is read from memory
-// (1 + 12/TT) times instead of 13 times.
+// (1 + 12/TT) times instead of 13 times.  The tile's rows of the band table go
+// through LDS too, loaded together with the vectors, so no global load is
+// waited for after the barrier.
 //
 // Update step B_i of preconditioned_cg_fast_test (hmc_sampler_batch.py:797-844):
 //   alpha = rz_old / Re(p^dag Op);  x += alpha p;  r -= alpha Op;  z = P r;
