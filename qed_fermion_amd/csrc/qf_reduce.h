@@ -58,21 +58,23 @@ __device__ __forceinline__ void sum_partials(const double* const (&src)[NV], int
 // sum_partials split in two so that the loads can be issued early: the first
 // NP loads of each lane of the first wave are made by load_partials (before
 // the caller's other loads), the rest and the sum by sum_partials_pf.  Same
-// summation order, hence the same result, as sum_partials.
+// summation order as sum_partials; the sum starts from the first partial
+// instead of 0.0 + it (equal except for the sign of a zero).  The loads are
+// unconditional (clamped index, masked at the sum) so that no wait is placed
+// right behind them.
 template <int NV, int NP>
 struct Partials { double v[NV][NP]; };
 
 template <int NV, int NP>
 __device__ __forceinline__ Partials<NV, NP> load_partials(const double* const (&src)[NV], int n) {
     const int lane = threadIdx.x & 63;
-    Partials<NV, NP> p;
+    Partials<NV, NP> p{};
+    if (threadIdx.x < 64 && n > 0) {   // uniform per wave
 #pragma unroll
-    for (int k = 0; k < NV; ++k)
+        for (int k = 0; k < NV; ++k)
 #pragma unroll
-        for (int j = 0; j < NP; ++j) {
-            const int i = lane + 64 * j;
-            p.v[k][j] = (threadIdx.x < 64 && i < n) ? src[k][i] : 0.0;
-        }
+            for (int j = 0; j < NP; ++j) p.v[k][j] = src[k][min(lane + 64 * j, n - 1)];
+    }
     return p;
 }
 
@@ -83,9 +85,9 @@ __device__ __forceinline__ void sum_partials_pf(const Partials<NV, NP>& p, const
     if (threadIdx.x < 64) {
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
-            double s = 0.0;
+            double s = lane < n ? p.v[k][0] : 0.0;
 #pragma unroll
-            for (int j = 0; j < NP; ++j)
+            for (int j = 1; j < NP; ++j)
                 if (lane + 64 * j < n) s += p.v[k][j];
             for (int i = lane + 64 * NP; i < n; i += 64) s += src[k][i];
             s = wave_sum(s);
