@@ -26,7 +26,7 @@ import os
 
 import torch
 
-from . import checkpoint, kernels, precon
+from . import checkpoint, fourier_accel, kernels, precon
 from .trajectory import TrajectoryEngine, couplings
 
 
@@ -82,6 +82,14 @@ class HmcSampler(object):
         self.tau_block_idx = 0
         self.tau_block_size = self.Ltau // self.max_tau_block_idx
         self.threshold_queue = [collections.deque(maxlen=5 * self.max_tau_block_idx) for _ in range(self.bs)]
+        # adaptive mass estimate (hmc_sampler_batch.py:268-285; the environment knobs of 20-80, 126)
+        debug = _env("debug", 0, int) != 0
+        self.mass_mode = _env("mass_mode", 0, int)
+        self._fa = fourier_accel.SigmaHatState(
+            self.bs, self.Lx, self.Ly, self.Ltau, self.device, self.dtype,
+            sigma_mini_batch_size=50 if debug else 1000, anneal_times=5 if debug else 10,
+            lmd=_env("lmd", 0.9, float), sig_min=_env("sig_min", 0.8, float), sig_max=_env("sig_max", 1.2, float),
+            lambda_=_env("lambda_", 0.05, float))
         # CG iteration gears (hmc_sampler_batch.py:305-309): one gear unless set otherwise
         self._MAX_ITERS_TO_CAPTURE = [self.max_iter] if self.dtau <= 0.1 and self.precon_on else [400]
 
@@ -218,6 +226,34 @@ class HmcSampler(object):
     def draw_momentum(self):
         """[bs, 2, Lx, Ly, Ltau] ~ N(0, m) (hmc_sampler_batch.py:1247-1252)."""
         return torch.randn(self.bs, 2, self.Lx, self.Ly, self.Ltau, device=self.device, dtype=self.dtype) * math.sqrt(self.m)
+
+    # ------------------------------------------- Fourier-accelerated momenta
+    # The reference's adaptive mass (fourier_accel.py): estimated when mass_mode != 0, never
+    # used by its trajectory, which draws draw_momentum() (hmc_sampler_batch.py:2389-2390).
+    sigma_hat = property(lambda self: self._fa.sigma_hat, lambda self, v: setattr(self._fa, "sigma_hat", v))
+    sigma_hat_cpu = property(lambda self: self._fa.sigma_hat_cpu, lambda self, v: setattr(self._fa, "sigma_hat_cpu", v))
+    boson_mean_cpu = property(lambda self: self._fa.boson_mean_cpu)
+    multiplier = property(lambda self: self._fa.multiplier)
+
+    def update_sigma_hat_cpu(self, boson_cpu, i):
+        """hmc_sampler_batch.py:1202-1210."""
+        fourier_accel.update_sigma_hat(self._fa, boson_cpu, i)
+
+    def apply_sigma_hat_cpu(self, i):
+        """hmc_sampler_batch.py:1133-1200."""
+        fourier_accel.apply_sigma_hat(self._fa, i, self.mass_mode)
+
+    def stabilize_sigma_hat(self, method="shrink", **kwargs):
+        """hmc_sampler_batch.py:1212-1245."""
+        self._fa.sigma_hat = fourier_accel.stabilize_sigma_hat(self._fa.sigma_hat, method, **kwargs)
+
+    def draw_momentum_fft(self):
+        """[bs, 2, Lx, Ly, Ltau] ~ N(0, F^-1 diag(1/sigma_hat) F) (hmc_sampler_batch.py:1254-1265)."""
+        return fourier_accel.draw_momentum_fft(self._fa)
+
+    def apply_m_inv(self, p):
+        """M^-1 p = F^-1 sigma_hat F p (hmc_sampler_batch.py:1267-1277)."""
+        return fourier_accel.apply_m_inv(self._fa, p)
 
     def draw_psudo_fermion(self):
         """randn(complex)/sqrt(2), [bs, Ltau*Ly*Lx] (hmc_sampler_batch.py:1279-1287)."""

@@ -265,3 +265,26 @@ def test_stream_copy(dev):
     c, d = torch.zeros(3, device=dev), torch.zeros(3, device=dev)
     with pytest.raises(K.QfError):
         K.stream_copy(c, d)
+
+
+def test_adaptive_mass_methods_on_device(dev):
+    """The sampler's adaptive-mass methods (hmc_sampler_batch.py:1133-1277) on the device equal the
+    same utilities on CPU tensors."""
+    from qed_fermion_amd import fourier_accel as FA
+    from qed_fermion_amd.hmc_sampler import HmcSampler
+    torch.manual_seed(3)
+    hmc = HmcSampler(Lx=4, Ltau=40, bs=2, precision="fp64", device=dev)
+    ref = FA.SigmaHatState(2, 4, 4, 40, torch.device("cpu"), torch.float64, sigma_mini_batch_size=3)
+    hmc._fa.sigma_mini_batch_size = 3
+    for i in range(4):
+        b = hmc.boson.cpu() + 0.1 * i * torch.randn(hmc.boson.shape, dtype=torch.float64)
+        hmc.update_sigma_hat_cpu(b, i)
+        FA.update_sigma_hat(ref, b, i)
+    assert torch.allclose(hmc.sigma_hat_cpu, ref.sigma_hat_cpu, rtol=1e-12, atol=1e-14)
+    hmc.apply_sigma_hat_cpu(3)
+    FA.apply_sigma_hat(ref, 3)
+    assert hmc.sigma_hat.is_cuda and torch.allclose(hmc.sigma_hat.cpu(), ref.sigma_hat, rtol=1e-10)
+    p = hmc.draw_momentum_fft()
+    assert p.shape == (2, 2, 4, 4, 40) and p.is_cuda
+    ref.sigma_hat = hmc.sigma_hat.cpu()
+    assert torch.allclose(hmc.apply_m_inv(p).cpu(), FA.apply_m_inv(ref, p.cpu()), rtol=1e-10, atol=1e-12)
