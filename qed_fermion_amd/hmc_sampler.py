@@ -386,8 +386,8 @@ class HmcSampler(object):
             accp = torch.rand(self.bs, device=self.device, dtype=torch.float64) < torch.exp(H_old - H_new)
             self.boson[accp] = boson_new[accp]
             thresholds = torch.minimum(torch.exp(H_old - H_new), torch.ones_like(H_old))
-            for b in range(self.bs):
-                self.threshold_queue[b].append(thresholds[b].item())
+            for b, th in enumerate(thresholds.tolist()):   # one device-to-host copy per sweep
+                self.threshold_queue[b].append(th)
             self.tau_block_idx += 1
         self.tau_block_idx = 0
         return self.boson, accp, cg_converge_iter, cg_r_err
