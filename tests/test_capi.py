@@ -74,6 +74,20 @@ def test_argument_validation_without_gpu():
     out = (ctypes.c_int * 4)()
     assert h.qf_get_tiling(0, 1, 36, 360, out) == 0 and out[0] in (2, 4, 6)
     assert h.qf_get_tiling(0, 1, 66, 660, out) == 0 and out[0] == 6
+    # the streaming copy validates before it launches anything
+    buf = (ctypes.c_char * 64)()
+    assert h.qf_stream_copy(None, None, 16, None) == _lib.QF_ERR_INVALID
+    assert h.qf_stream_copy(ctypes.addressof(buf), ctypes.addressof(buf), 12, None) == _lib.QF_ERR_INVALID
+    assert "multiples of 16" in h.qf_last_error().decode()
+    assert h.qf_stream_copy(ctypes.addressof(buf), ctypes.addressof(buf), 0, None) == _lib.QF_OK
+
+
+def test_paired_x_switch():
+    from qed_fermion_amd import _lib
+    h = _lib.lib()
+    assert h.qf_set_paired_x_updates(0) == 1          # on by default
+    assert h.qf_set_paired_x_updates(5) == 0          # any nonzero value turns it on
+    assert h.qf_set_paired_x_updates(1) == 1
 
 
 def test_ragged_plan_layout():
