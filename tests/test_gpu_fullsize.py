@@ -101,3 +101,30 @@ def test_mixed_vs_fp64_c2(dev):
         s64 = np.vdot(bn[i], x64n[i]).real
         sm = np.vdot(bn[i], xmn[i]).real
         assert abs(sm - s64) < 1e-7 * abs(s64)
+
+
+@pytest.mark.parametrize("L,Lt,nrhs", [(36, 360, 2), (66, 660, 1)])
+def test_mixed_vs_fp64_full_size(dev, L, Lt, nrhs):
+    """The precision configuration's claim at the headline size (L=36, Ltau=360) and at L=66: on a
+    per-bond Gaussian field (sigma 0.3) both solves reach 1e-10 as recomputed by the oracle, the
+    mixed solution and force are within 1e-7 of FP64, and so is S_f."""
+    k = _k()
+    print(f"[mixed full] L={L} Lt={Lt} nrhs={nrhs}", flush=True)
+    phi = make_phi(L, Lt, "gauss", seed=L + 5, nchain=nrhs)
+    R = make_vec(L, Lt, seed=L + 6, nrhs=nrhs)
+    phid = _t(phi.reshape(nrhs, -1), torch.float64)
+    b = k.apply("Mdag", phid, _t(R, torch.complex128), L, Lt)
+    ptab = k.precon_table(Lt, device="cuda")
+    x64, it64, e64 = k.pcg_solve(phid, b, L, Lt, ptab=ptab, rtol=1e-10, max_iter=4000)
+    xm, itm, em = k.pcg_mixed_solve(phid, b, L, Lt, ptab=ptab, rtol=1e-10, max_iter=4000)
+    print(f"[mixed full] iterations fp64 {it64.tolist()} mixed {itm.tolist()}", flush=True)
+    bn, x64n, xmn = b.cpu().numpy(), x64.cpu().numpy(), xm.cpu().numpy()
+    F64 = k.fermion_force(phid, x64, L, Lt)
+    Fm = k.fermion_force(phid, xm, L, Lt)
+    for i in range(nrhs):
+        assert relerr(O.apply_MdagM(phi[i], x64n[i], L, Lt), bn[i]) <= 1.01e-10
+        assert relerr(O.apply_MdagM(phi[i], xmn[i], L, Lt), bn[i]) <= 1.01e-10
+        assert relerr(xmn[i], x64n[i]) < 1e-7
+        assert relerr(Fm[i].cpu().numpy(), F64[i].cpu().numpy()) < 1e-7
+        s64 = np.vdot(bn[i], x64n[i]).real
+        assert abs(np.vdot(bn[i], xmn[i]).real - s64) < 1e-7 * abs(s64)
