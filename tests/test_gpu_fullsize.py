@@ -128,3 +128,32 @@ def test_mixed_vs_fp64_full_size(dev, L, Lt, nrhs):
         assert relerr(Fm[i].cpu().numpy(), F64[i].cpu().numpy()) < 1e-7
         s64 = np.vdot(bn[i], x64n[i]).real
         assert abs(np.vdot(bn[i], xmn[i]).real - s64) < 1e-7 * abs(s64)
+
+
+def test_trajectory_mixed_vs_fp64_c3(dev):
+    """A trajectory at the headline size (L=36, Ltau=360, per-bond Gaussian field, 2 leapfrog steps,
+    every solve to 1e-10) from the same (phi, p, R) in mixed precision and in FP64: final field and
+    momenta within 1e-7, H0 within 1e-9 relative and Delta H within 1e-4 absolute (the FP64 path
+    follows the oracle's trajectory at the smaller configurations)."""
+    from qed_fermion_amd.trajectory import TrajectoryEngine
+    L, Lt = 36, 360
+    print("[trajectory C3]", flush=True)
+    phi = make_phi(L, Lt, "gauss", seed=3601)
+    p = np.random.default_rng(3602).standard_normal(phi.shape)
+    R = make_vec(L, Lt, seed=3603)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        eng = TrajectoryEngine(L, Lt, 1, precision=prec, max_iter=2000, rtol=1e-10, n_leap=2, delta_t=0.03, device="cuda")
+        eng.set_phi(torch.from_numpy(phi.reshape(1, -1)).cuda())
+        eng.p.copy_(torch.from_numpy(p.reshape(1, -1)))
+        eng.R.copy_(torch.from_numpy(R))
+        eng.propose(draw=False)
+        torch.cuda.synchronize()
+        out[prec] = (eng.x.cpu().numpy(), eng.p.cpu().numpy(), eng.H0.item(), eng.H1.item(), eng.err.max().item())
+    x64, p64, H0, H1, e64 = out["fp64"]
+    xm, pm, H0m, H1m, em = out["mixed"]
+    print(f"[trajectory C3] dH fp64 {H1 - H0:.6f} mixed {H1m - H0m:.6f}", flush=True)
+    assert e64 <= 1e-10 and em <= 1.01e-10
+    assert relerr(xm, x64) < 1e-7 and relerr(pm, p64) < 1e-7
+    assert abs(H0m - H0) < 1e-9 * abs(H0)
+    assert abs((H1m - H0m) - (H1 - H0)) < 1e-4     # absolute, on |Delta H| of order 1..1000
