@@ -157,3 +157,31 @@ def test_trajectory_mixed_vs_fp64_c3(dev):
     assert relerr(xm, x64) < 1e-7 and relerr(pm, p64) < 1e-7
     assert abs(H0m - H0) < 1e-9 * abs(H0)
     assert abs((H1m - H0m) - (H1 - H0)) < 1e-4     # absolute, on |Delta H| of order 1..1000
+
+
+def test_multi_rhs_solve_c4(dev):
+    """The batching configuration's solve: 128 complex64 Z2 noise vectors at L=48, Ltau=480 on one
+    shared field in one multi-RHS PCG (per-bond Gaussian field, 1e-5).  Every reported error is
+    below rtol; the oracle recomputes the true residual of the first and last solution, and of a
+    solve of those two vectors alone (batching does not mix right-hand sides)."""
+    k = _k()
+    L, Lt, nrhs = 48, 480, 128
+    print("[multi-rhs C4]", flush=True)
+    N = L * L * Lt
+    phi = make_phi(L, Lt, "gauss", seed=4801)
+    rng = np.random.default_rng(4802)
+    eta = (((rng.integers(0, 2, (nrhs, N)) * 2 - 1) + 1j * (rng.integers(0, 2, (nrhs, N)) * 2 - 1)) / np.sqrt(2))
+    phid = _t(phi.reshape(1, -1), torch.float32)
+    ptab = k.precon_table(Lt, device="cuda", dtype=torch.float32)
+    b = k.apply("Mdag", phid, _t(eta, torch.complex64), L, Lt)
+    x, it, err = k.pcg_solve(phid, b, L, Lt, ptab=ptab, rtol=1e-5, max_iter=1500)
+    print(f"[multi-rhs C4] iterations {it.min().item():.0f}..{it.max().item():.0f}, max err {err.max().item():.2e}", flush=True)
+    assert err.max().item() <= 1e-5
+    pick = [0, nrhs - 1]
+    x2, _, _ = k.pcg_solve(phid, b[pick].contiguous(), L, Lt, ptab=ptab, rtol=1e-5, max_iter=1500)
+    xn, x2n, bn = x.cpu().numpy(), x2.cpu().numpy(), b.cpu().numpy()
+    for j, i in enumerate(pick):
+        bi = bn[i].astype(np.complex128)
+        for sol in (xn[i], x2n[j]):
+            res = relerr(O.apply_MdagM(phi[0], sol.astype(np.complex128), L, Lt), bi)
+            assert res < 3e-5, (i, res)
