@@ -77,13 +77,16 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=0, help="oracle PCG iterations for cpu_baseline (0 = auto, ~15 s)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="use the built-in tilings instead of the autotuner")
-    ap.add_argument("--probe", choices=["per_chain", "shared", "bond_bond"], default=None,
+    ap.add_argument("--probe", choices=["per_chain", "shared", "bond_bond", "pcg"], default=None,
                     help="per_chain / shared: only launch a calibration copy and the roofline M^dag M three times each "
                          "(for rocprofv3 --pmc); bond_bond: only the bond-bond estimators of one chain at L=36, "
-                         "Ltau=360, Nrv=40, twice (for rocprofv3)")
+                         "Ltau=360, Nrv=40, twice (for rocprofv3); pcg: a calibration copy, then 20 iterations of "
+                         "the headline mixed PCG (--bs chains, own fields, no chain stops)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batching (L=48, 128 Z2 RHS; ragged set) and L=66 trajectory measurements")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--tiling", default=None,
+                    help="with --probe pcg: patch height, tau run and update tile as BY,T,TB instead of the autotuner")
     return ap.parse_args()
 
 
@@ -279,6 +282,38 @@ def stream_copy_rate(torch, K, dev, reps):
             "ms_per_launch": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}
 
 
+def probe_pcg(args):
+    """Three launches of a plain copy of known size, then one mixed-precision PCG solve of 20
+    iterations at rtol 0 (so that no chain stops) on --bs chains of the headline field, tuned as in
+    the full run: under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE the copy's counters give the scale
+    of the two PCG kernels' (step A, step B held and applying)."""
+    import numpy as np
+    import torch
+    from qed_fermion_amd import autotune
+    from qed_fermion_amd import kernels as K
+    L, Lt, bs, dev = args.L, args.Ltau, args.bs, torch.device("cuda:0")
+    N = L * L * Lt
+    if args.tiling:
+        by, t, tb = (int(v) for v in args.tiling.split(","))
+        K.set_tuning_shape(0, L, Lt, bs, by, t, tb)
+    elif not args.no_tune:
+        autotune.tune(0, L, Lt, bs, kind="pcg", persist=False)
+    phi = torch.from_numpy(make_field(np, args.field, np.random.default_rng(args.seed), bs, N)).to(dev)
+    g = torch.Generator(device="cpu").manual_seed(9)
+    b = torch.randn(bs, N, generator=g, dtype=torch.complex128).to(dev)
+    ptab = K.precon_table(Lt, device=dev)
+    xc = torch.empty(bs * N * 2, dtype=torch.float32, device=dev).uniform_()
+    yc = torch.empty_like(xc)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        yc.copy_(xc)
+    torch.cuda.synchronize()
+    K.pcg_mixed_solve(phi, b, L, Lt, ptab=ptab, rtol=0.0, max_iter=20, ru_period=args.ru_period)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "pcg", "L": L, "Ltau": Lt, "chains": bs, "iterations": 20,
+                      "copy_bytes_each_way": xc.numel() * 4, "tiling": K.get_tiling(0, bs, L, Lt)}), flush=True)
+
+
 def probe_bond_bond():
     """The bond-bond estimators of one chain (L=36, Ltau=360, Nrv=40, complex64, synthetic
     G_eta), evaluated twice; prints the time of the second evaluation."""
@@ -440,6 +475,9 @@ def main():
         sys.exit(2)
     if args.probe == "bond_bond":
         probe_bond_bond()
+        return
+    if args.probe == "pcg":
+        probe_pcg(args)
         return
     if args.probe:
         probe(args)
