@@ -174,6 +174,14 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
         }
     };
 
+    if constexpr (MODE == MDM_CG) {
+        // a chain that stopped in an earlier iteration carries its state and loads nothing
+        const CgState S0 = A.st_prev[rho];
+        if (S0.act == 0.0) {   // uniform over the block
+            if (run == 0 && threadIdx.x == 0) A.st_out[rho] = S0;
+            return;
+        }
+    }
     // prologue loads first: they do not depend on the CG scalars (the partial sums
     // those come from are requested before them)
     Partials<2, kMdmPartialPrefetch> pf{};
