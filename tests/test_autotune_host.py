@@ -1,0 +1,20 @@
+"""Host logic of the autotuner (no GPU): the re-timing of the best candidates."""
+from qed_fermion_amd import autotune
+
+
+def test_refine_retimes_the_best_candidates_and_keeps_minima(monkeypatch):
+    samples = iter([0.9, 1.2, 1.5, 1.0, 0.8, 1.6])      # rounds of (a, b, c), interleaved
+    order = []
+    monkeypatch.setattr(autotune, "_time", lambda fn, reps, warm=2: next(samples))
+    table = {"a": 1.0, "b": 1.1, "c": 2.0, "d": 3.0}
+    autotune._refine(table, order.append, lambda: None, reps=1, top=3, rounds=2)
+    assert order == ["a", "b", "c", "a", "b", "c"]       # the slowest candidate is not re-timed
+    assert table == {"a": 0.9, "b": 0.8, "c": 1.5, "d": 3.0}
+    assert min(table, key=table.get) == "b"
+
+
+def test_refine_scales_per_iteration_times(monkeypatch):
+    monkeypatch.setattr(autotune, "_time", lambda fn, reps, warm=2: 20.0)
+    table = {"x": 5.0, "y": 0.5}
+    autotune._refine(table, lambda k: None, lambda: None, reps=1, scale=1.0 / 20, top=2, rounds=1)
+    assert table == {"x": 1.0, "y": 0.5}
