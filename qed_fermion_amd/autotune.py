@@ -61,7 +61,7 @@ def valid_by(L, prec):
 
 
 def _t_candidates(Lt):
-    return sorted({t for t in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 14, 16, 19, 22) if t <= Lt})
+    return sorted({t for t in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 14, 15, 16, 19, 22, 24) if t <= Lt})
 
 
 def _time(fn, reps, warm=2):
