@@ -86,7 +86,8 @@ def parse():
                     help="skip the batching (L=48, 128 Z2 RHS; ragged set) and L=66 trajectory measurements")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--tiling", default=None,
-                    help="with --probe pcg: patch height, tau run and update tile as BY,T,TB instead of the autotuner")
+                    help="with --probe pcg: patch height, tau run and update tile as BY,T,TB instead of the autotuner; "
+                         "with --probe per_chain / shared: one or more such tilings separated by ';', three launches each")
     return ap.parse_args()
 
 
@@ -122,7 +123,8 @@ def mdagm_roofline(torch, K, L, Lt, nrhs, reps, dev, use_hop=True):
     algo = nrhs * 16 * N + field
     return {"kernel": "mdagm_kernel<float> (M^dag M, complex64, one field shared by the right-hand sides, "
                       + ("from the hop table" if use_hop else "from phi") + ")",
-            "nrhs": nrhs, "ms_per_launch": ms, "bytes_per_launch": algo, "GBps": algo / (ms * 1e-3) / 1e9}
+            "nrhs": nrhs, "ms_per_launch": ms, "bytes_per_launch": algo, "GBps": algo / (ms * 1e-3) / 1e9,
+            "tiling": K.get_tiling(0, nrhs, L, Lt)}
 
 
 def timed(torch, fn, reps=1):
@@ -209,11 +211,12 @@ def extras(torch, K, args, dev):
     return out
 
 
-def traffic_of(L, Lt, nrhs, kind):
-    """HBM bytes per launch of an M^dag M measurement at this shape, as
+def traffic_of(L, Lt, nrhs, kind, tiling):
+    """HBM bytes per launch of an M^dag M measurement at this shape and tiling (patch height and
+    tau run; the halo re-reads depend on both), as
     measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (each scaled by the
     same counter's reading for a copy of known size) and recorded in
-    profiles/mdagm_traffic.json; None if that shape was not measured."""
+    profiles/mdagm_traffic.json; None if that shape and tiling were not measured."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "mdagm_traffic.json")
     try:
         with open(path) as f:
@@ -221,7 +224,8 @@ def traffic_of(L, Lt, nrhs, kind):
     except (OSError, ValueError):
         return None
     for e in rec.get("entries", []):
-        if (e.get("L"), e.get("Ltau"), e.get("nrhs"), e.get("kind")) == (L, Lt, nrhs, kind):
+        if (e.get("L"), e.get("Ltau"), e.get("nrhs"), e.get("kind")) == (L, Lt, nrhs, kind) and \
+                (e["tiling"]["by"], e["tiling"]["t"]) == (tiling["by"], tiling["t"]):
             return e.get("traffic_bytes")
     return None
 
@@ -335,7 +339,8 @@ def probe_bond_bond():
 def probe(args):
     """Three launches of a plain copy of known size, then three of the roofline
     M^dag M (per-chain fields: --roof-chains chains; shared: --roof-nrhs RHS on
-    one field), tuned as in the full run.  Under rocprofv3 --pmc FETCH_SIZE /
+    one field), tuned as in the full run, or three for each tiling of --tiling
+    "BY,T,TB;BY,T,TB;..." in that order.  Under rocprofv3 --pmc FETCH_SIZE /
     WRITE_SIZE the copy's counters give the scale of the kernel's."""
     import torch
     from qed_fermion_amd import autotune
@@ -344,7 +349,8 @@ def probe(args):
     N = L * L * Lt
     shared = args.probe == "shared"
     n = args.roof_nrhs if shared else args.roof_chains
-    if not args.no_tune:
+    tilings = [tuple(int(v) for v in tl.split(",")) for tl in args.tiling.split(";")] if args.tiling else [None]
+    if tilings[0] is None and not args.no_tune:
         autotune.tune(0, L, Lt, n, kind="operator", shared_field=shared, persist=False)
     g = torch.Generator(device="cpu").manual_seed(7)
     phi = ((torch.rand(1 if shared else n, 2 * N, generator=g, dtype=torch.float64) * 2 - 1) * math.pi).to(dev, torch.float32)
@@ -357,12 +363,17 @@ def probe(args):
     for _ in range(3):
         yc.copy_(xc)
     torch.cuda.synchronize()
-    for _ in range(3):
-        K.apply_hop("MdagM", hop, x, L, Lt, out=y)
-    torch.cuda.synchronize()
+    order = []
+    for tl in tilings:
+        if tl is not None:
+            K.set_tuning_shape(0, L, Lt, n, *tl)
+        for _ in range(3):
+            K.apply_hop("MdagM", hop, x, L, Lt, out=y)
+        torch.cuda.synchronize()
+        order.append(K.get_tiling(0, n, L, Lt))
     print(json.dumps({"probe": args.probe, "L": L, "Ltau": Lt, "n": n, "copy_bytes_each_way": xc.numel() * 4,
                       "algo_read": n * 8 * N + (1 if shared else n) * 16 * N, "algo_write": n * 8 * N,
-                      "tiling": K.get_tiling(0, n, L, Lt)}), flush=True)
+                      "tilings_in_launch_order": order}), flush=True)
 
 
 def make_field(np, kind, rng, bs, N):
@@ -595,14 +606,15 @@ def main():
         "mdagm_per_chain": per_chain,
         "other_configs": extra,
         "roofline": {"bound": "hbm", "achieved": per_chain["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": per_chain["GBps"] / HBM_PEAK_GBS, "traffic": traffic_of(L, Lt, per_chain["nchains"], "per_chain"),
+                     "frac": per_chain["GBps"] / HBM_PEAK_GBS, "traffic": traffic_of(L, Lt, per_chain["nchains"], "per_chain", per_chain["tiling"]),
                      "kernel": per_chain["kernel"], "nchains": per_chain["nchains"],
                      "bytes_per_launch": per_chain["bytes_per_launch"], "ms_per_launch": per_chain["ms_per_launch"],
                      "tiling": per_chain["tiling"], "frac_of_stream_copy": per_chain["GBps"] / copy["GBps"]},
         "stream_copy": copy,
         "mdagm_shared_field": {"GBps": roof["GBps"], "frac": roof["GBps"] / HBM_PEAK_GBS, "kernel": roof["kernel"],
                                "nrhs": roof["nrhs"], "bytes_per_launch": roof["bytes_per_launch"],
-                               "ms_per_launch": roof["ms_per_launch"], "traffic": traffic_of(L, Lt, roof["nrhs"], "shared")},
+                               "ms_per_launch": roof["ms_per_launch"], "tiling": roof["tiling"],
+                               "traffic": traffic_of(L, Lt, roof["nrhs"], "shared", roof["tiling"])},
         "tilings": tilings,
         "cpu_baseline": cpu,
     }
