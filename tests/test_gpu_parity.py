@@ -304,3 +304,43 @@ def test_operator_repeat_multiwave(dev, prec, tol, L, by):
                         assert relerr(yn[i], ref[i]) < tol, (T, i)
     finally:
         k.set_tuning(pi, 0, 0, 0)
+
+
+@pytest.mark.parametrize("T", [12, 15, 16, 24, 30])
+def test_long_tau_runs(dev, T):
+    """tau runs of 12 to 30 slices, the autotuner's longest candidates, on Ntau = 50 (so that
+    every run length but 25 leaves a shorter last run): M^dag M on per-chain fields from phi
+    and from the hop table in both precisions against the oracle, the FP64 PCG against the
+    oracle's recurrence, and the mixed solver to its tolerance in the oracle's true residual."""
+    k = _k()
+    L, Lt, nrhs = 6, 50, 2
+    print(f"[long tau runs] T={T}", flush=True)
+    phi = make_phi(L, Lt, "gauss", seed=T, nchain=nrhs)
+    x = make_vec(L, Lt, seed=T + 1, nrhs=nrhs)
+    ref = [O.apply_MdagM(phi[i], x[i], L, Lt) for i in range(nrhs)]
+    try:
+        for pi, rd, cd, tol in ((1, torch.float64, torch.complex128, 1e-12), (0, torch.float32, torch.complex64, 2e-5)):
+            k.set_tuning(pi, 0, T, 0)
+            pd, xd = _t(phi.reshape(nrhs, -1), dev, rd), _t(x, dev, cd)
+            hop = k.hop_table(pd, L, Lt)
+            for y in (k.apply("MdagM", pd, xd, L, Lt), k.apply_hop("MdagM", hop, xd, L, Lt)):
+                yn = y.cpu().numpy()
+                for i in range(nrhs):
+                    assert relerr(yn[i], ref[i]) < tol, (pi, i, relerr(yn[i], ref[i]))
+        tab = O.precon_table(Lt)
+        b = np.stack([O.apply_Mdag(phi[i], x[i], L, Lt) for i in range(nrhs)])
+        phd, bd, tbd = _t(phi.reshape(nrhs, -1), dev, torch.float64), _t(b, dev, torch.complex128), _t(tab, dev, torch.float64)
+        xs, iters, _ = k.pcg_solve(phd, bd, L, Lt, ptab=tbd, rtol=1e-9, max_iter=80)
+        xs = xs.cpu().numpy()
+        for i in range(nrhs):
+            xr, it, _ = O.pcg(phi[i], b[i], L, Lt, table=tab, rtol=1e-9, max_iter=80, early_exit=False)
+            assert relerr(xs[i], xr) < 1e-8, relerr(xs[i], xr)
+            assert int(iters[i].item()) == it
+        xm, _, em = k.pcg_mixed_solve(phd, bd, L, Lt, ptab=tbd, rtol=1e-10, max_iter=200, ru_period=50)
+        xm = xm.cpu().numpy()
+        for i in range(nrhs):
+            true_res = relerr(O.apply_MdagM(phi[i], xm[i], L, Lt), b[i])
+            assert true_res < 2e-10 and em[i].item() < 1e-10, (true_res, em[i].item())
+    finally:
+        k.set_tuning(1, 0, 0, 0)
+        k.set_tuning(0, 0, 0, 0)
