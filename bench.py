@@ -480,7 +480,7 @@ def accuracy(torch, K, TrajectoryEngine, args, dev, phi_s, max_iter, rtol):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only (--gpus must be 1): chains, pseudofermions and "
+        print(json.dumps({"error": "qed_fermion_amd runs on one GPU only (--gpus must be 1): chains, pseudofermions and "
                                    "noise vectors would shard across GPUs without communication, but no multi-GPU "
                                    "path is built"}))
         sys.exit(2)
