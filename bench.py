@@ -282,8 +282,27 @@ def stream_copy_rate(torch, K, dev, reps):
     ms = e0.elapsed_time(e1) / reps
     if not torch.equal(a, b):
         raise RuntimeError("stream copy produced a wrong result")
+    # the same bytes with one work-group per chunk, each copying its chunk front to back: as many
+    # concurrent address streams as the roofline M^dag M's 480-1280 work-groups run
+    chunked = {}
+    for nch in (480, 960, 1920):
+        chunk = (nbytes // nch) // 16 * 16
+        b.zero_()
+        for _ in range(3):
+            K.stream_copy_chunks(a, b, chunk)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(reps):
+            K.stream_copy_chunks(a, b, chunk)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        msc = e0.elapsed_time(e1) / reps
+        if not torch.equal(a, b):
+            raise RuntimeError("chunked stream copy produced a wrong result")
+        chunked[str(nch)] = {"chunk_bytes": chunk, "ms_per_launch": msc, "GBps": 2 * nbytes / (msc * 1e-3) / 1e9}
     return {"kernel": "stream_copy_kernel (one 16-byte element per thread)", "bytes_each_way": nbytes,
-            "ms_per_launch": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}
+            "ms_per_launch": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9,
+            "chunked": {"kernel": "chunk_copy_kernel (one work-group per chunk, front to back)", "by_chunks": chunked}}
 
 
 def probe_pcg(args):
@@ -609,7 +628,8 @@ def main():
                      "frac": per_chain["GBps"] / HBM_PEAK_GBS, "traffic": traffic_of(L, Lt, per_chain["nchains"], "per_chain", per_chain["tiling"]),
                      "kernel": per_chain["kernel"], "nchains": per_chain["nchains"],
                      "bytes_per_launch": per_chain["bytes_per_launch"], "ms_per_launch": per_chain["ms_per_launch"],
-                     "tiling": per_chain["tiling"], "frac_of_stream_copy": per_chain["GBps"] / copy["GBps"]},
+                     "tiling": per_chain["tiling"], "frac_of_stream_copy": per_chain["GBps"] / copy["GBps"],
+                     "frac_of_chunked_copy": per_chain["GBps"] / max(c["GBps"] for c in copy["chunked"]["by_chunks"].values())},
         "stream_copy": copy,
         "mdagm_shared_field": {"GBps": roof["GBps"], "frac": roof["GBps"] / HBM_PEAK_GBS, "kernel": roof["kernel"],
                                "nrhs": roof["nrhs"], "bytes_per_launch": roof["bytes_per_launch"],

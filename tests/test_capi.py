@@ -80,6 +80,11 @@ def test_argument_validation_without_gpu():
     assert h.qf_stream_copy(ctypes.addressof(buf), ctypes.addressof(buf), 12, None) == _lib.QF_ERR_INVALID
     assert "multiples of 16" in h.qf_last_error().decode()
     assert h.qf_stream_copy(ctypes.addressof(buf), ctypes.addressof(buf), 0, None) == _lib.QF_OK
+    assert h.qf_stream_copy_chunks(None, None, 16, 16, None) == _lib.QF_ERR_INVALID
+    assert h.qf_stream_copy_chunks(ctypes.addressof(buf), ctypes.addressof(buf), 32, 0, None) == _lib.QF_ERR_INVALID
+    assert h.qf_stream_copy_chunks(ctypes.addressof(buf), ctypes.addressof(buf), 32, 24, None) == _lib.QF_ERR_INVALID
+    assert "multiples of 16" in h.qf_last_error().decode()
+    assert h.qf_stream_copy_chunks(ctypes.addressof(buf), ctypes.addressof(buf), 0, 16, None) == _lib.QF_OK
 
 
 def test_paired_x_switch():

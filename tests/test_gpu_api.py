@@ -265,6 +265,15 @@ def test_stream_copy(dev):
     c, d = torch.zeros(3, device=dev), torch.zeros(3, device=dev)
     with pytest.raises(K.QfError):
         K.stream_copy(c, d)
+    # chunked: chunks that split the buffer unevenly (a shorter last chunk), chunks longer than
+    # one step of a work-group (several steps, a partial last step), and one chunk for all
+    for chunk in (16, 48, 4096 * 16 + 32, 16400, 1 << 20):
+        b.zero_()
+        K.stream_copy_chunks(a, b, chunk)
+        torch.cuda.synchronize()
+        assert torch.equal(a, b), chunk
+    with pytest.raises(K.QfError):
+        K.stream_copy_chunks(a, b, 24)
 
 
 def test_adaptive_mass_methods_on_device(dev):
