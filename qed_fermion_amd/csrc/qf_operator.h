@@ -182,6 +182,18 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
             return;
         }
     }
+    // Pipelined coefficients from phi (PHIPF): the angles of slice tau+2 are loaded in step
+    // tau and their sincos is taken at the start of step tau+1, so that no step waits for a
+    // load it has issued itself.  From the hop table (HOPT) the coefficients are loaded
+    // straight into registers one or two steps ahead.
+    constexpr bool PHIPF = PIPE && !HOPT;
+    constexpr bool REGPF = PIPE && HOPT && mdm_regpf<R, BY, MODE>();
+    constexpr bool TOUCH = !PHIPF && !REGPF;
+    // coefficient source of one slice: hop entries or phi
+    const unsigned cslice = HOPT ? 2u * sstride : 2u * (unsigned)Vs * (unsigned)sizeof(R), nlanes = blockDim.x;
+    const __amdgpu_buffer_rsrc_t rs_c = HOPT ? rs_h : buf_rsrc(phi, cslice * (unsigned)Lt);
+    auto ldphi = [&](cplx<R> (&q)[2 * BY], int tau) { load_patch_buf<R, BY>(q, rs_c, po, (unsigned)tau * cslice, L); };
+
     // prologue loads first: they do not depend on the CG scalars (the partial sums
     // those come from are requested before them)
     Partials<2, kMdmPartialPrefetch> pf{};
@@ -193,10 +205,19 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
     const int t0n = (t0 + 1 == Lt) ? 0 : t0 + 1;
     CoefC<R, BY> c0, c1, c2;
     Raw rw, rx;
-    coef(c2, tm);
-    coef(c0, t0);
+    cplx<R> qa[PHIPF ? 2 * BY : 1], qb[PHIPF ? 2 * BY : 1], qn[PHIPF ? 2 * BY : 1];   // angles of tm, t0, t0+1
+    if constexpr (PHIPF) {
+        ldphi(qa, tm);
+        ldphi(qb, t0);
+    } else {
+        coef(c2, tm);
+        coef(c0, t0);
+    }
     ldraw(rw, tm);
     ldraw(rx, t0);
+    if constexpr (PHIPF) {
+        if (t0 + 1 < t1) ldphi(qn, t0n);
+    }
 
     if constexpr (MODE == MDM_CG) {
         // Scalars of the previous iteration, reduced in a fixed order by
@@ -227,6 +248,10 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
         }
         if (act == 0.0) return;   // uniform over the block
         beta = (R)bt;
+    }
+    if constexpr (PHIPF) {
+        coefc_from_angles<R, BY>(c2, qa, A.hp);
+        coefc_from_angles<R, BY>(c0, qb, A.hp);
     }
 
     const __amdgpu_buffer_rsrc_t rs_b = buf_rsrc(MODE == MDM_RESID ? (const void*)(A.b + vo) : (const void*)A.x, MODE == MDM_RESID ? vbytes : 0u);
@@ -262,12 +287,8 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
         }
     };
 
-    constexpr bool REGPF = PIPE && mdm_regpf<R, BY, MODE>();
     constexpr int NTOUCH = (BY * (int)S + 31) / 32;   // 128-byte lines of a hop slice per lane
     unsigned tacc = 0, td[NTOUCH];
-    // coefficient source of one slice: hop entries or phi
-    const unsigned cslice = HOPT ? 2u * sstride : 2u * (unsigned)Vs * (unsigned)sizeof(R), nlanes = blockDim.x;
-    const __amdgpu_buffer_rsrc_t rs_c = HOPT ? rs_h : buf_rsrc(phi, cslice * (unsigned)Lt);
     if constexpr (!PIPE) {
         // x_{tau+1} is not kept across the second B: it is read again (an L2
         // hit: MODE_CG from the p just stored, else from x) while W is being
@@ -337,7 +358,7 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
     for (int k = 0; k < 2 * BY; ++k) W[k] = X[k];
     if constexpr (REGPF) {
         if (t0 + 1 < t1) coef(c1, t0 + 1);
-    } else {
+    } else if constexpr (TOUCH) {
         touch_lines<NTOUCH>(td, rs_c, (unsigned)t0n * cslice, cslice, nlanes);
     }
     ldraw(rx, t0n);   // x_{t0+1}
@@ -346,6 +367,45 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
     make(X, rx);
     axpy_patch<R, BY>(Yn, X, sgnM(t0n) * cn, W);
 
+    if constexpr (PHIPF) {
+    // one step; cA = C_tau, cB receives C_{tau+1} from the angles qn loaded one step earlier
+    auto stepq = [&](int tau, CoefC<R, BY>& cA, CoefC<R, BY>& cB) {
+        const bool more = tau + 1 < t1;
+        const int tn = (tau + 1 == Lt) ? 0 : tau + 1;
+        const int tnn = (tn + 1 == Lt) ? 0 : tn + 1;
+        cplx<R> bb[2 * BY];
+#pragma unroll
+        for (int k = 0; k < 2 * BY; ++k) W[k] = Yn[k];
+        if (more) {
+#pragma unroll
+            for (int k = 0; k < 2 * BY; ++k) U[k] = X[k];
+            if constexpr (MODE == MDM_CG) {
+                if (g.valid) store_patch_g<R, BY>(X, pnew + (long long)tn * Vs, po, L);
+            }
+            coefc_from_angles<R, BY>(cB, qn, A.hp);           // C_{tau+1}
+            // angles of slice tau+2; in the last step of the run they are not used, and slice
+            // tau+1 (just read, an L2 hit) is loaded instead so that the load is unconditional
+            ldphi(qn, tau + 2 < t1 ? tnn : tn);
+            ldraw(rx, tnn);                                   // x_{tau+2} (the halo slice at the end of the run)
+            apply_B2c<R, BY>(W, cA, U, cB, lds, xa, phase);   // W = B_tau y_{tau+1}, U = B_{tau+1} x_{tau+1}
+        } else {
+            apply_Bc<R, BY>(W, cA, lds, xa, phase);
+        }
+        emit(tau, bb, W);
+        if (more) {
+#pragma unroll
+            for (int k = 0; k < 2 * BY; ++k) Y[k] = Yn[k];
+            make(X, rx);                                      // x_{tau+2}
+            axpy_patch<R, BY>(Yn, X, sgnM(tnn) * cn, U);      // y_{tau+2}
+        }
+    };
+    for (int tau = t0;;) {
+        stepq(tau, c0, c1);
+        if (++tau >= t1) break;
+        stepq(tau, c1, c0);
+        if (++tau >= t1) break;
+    }
+    } else {
     // one step; cA = C_tau, cB = C_{tau+1}, cC receives C_{tau+2}
     auto step = [&](int tau, CoefC<R, BY>& cA, CoefC<R, BY>& cB, CoefC<R, BY>& cC) {
         const bool more = tau + 1 < t1;
@@ -390,9 +450,10 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
         step(tau, c2, c0, c1);
         if (++tau >= t1) break;
     }
+    }
 
     }
-    if constexpr (!REGPF) {
+    if constexpr (TOUCH) {
         // fold the touched dwords in; T < 0 never holds, so this only keeps the loads alive
 #pragma unroll
         for (int k = 0; k < NTOUCH; ++k) tacc ^= td[k];

@@ -453,6 +453,22 @@ __device__ __forceinline__ void load_coefc(CoefC<R, BY>& cf, const R* __restrict
     }
 }
 
+// The same coefficients from the raw angles of the patch, q[2r] = (phi_x, phi_y) of site
+// (2a, y0+r) and q[2r+1] those of (2a+1, y0+r): the layout load_patch_buf gives when it
+// reads a phi slice, so the angles can be loaded one step before the sincos is taken.
+template <typename R, int BY>
+__device__ __forceinline__ void coefc_from_angles(CoefC<R, BY>& cf, const cplx<R> (&q)[2 * BY], const Hop<R>& hp) {
+#pragma unroll
+    for (int r = 0; r < BY; ++r) {
+        const R t_in = (r & 1) ? hp.th2 : hp.th1;
+        const R t_ed = (r & 1) ? hp.th1 : hp.th2;
+        cf.ex[r][0] = phase<R>(t_in, q[2 * r].x);
+        cf.ex[r][1] = phase<R>(t_ed, q[2 * r + 1].x);
+        cf.ey[0][r] = phase<R>(hp.th2, q[2 * r].y);
+        cf.ey[1][r] = phase<R>(hp.th2, q[2 * r + 1].y);
+    }
+}
+
 // conj(h) * u
 template <typename R> __device__ __forceinline__ cplx<R> cjmul_h(cplx<R> h, cplx<R> u) {
     return mk<R>(fma(h.x, u.x, h.y * u.y), fma(h.x, u.y, -h.y * u.x));
