@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define QF_ABI_VERSION 1
+#define QF_ABI_VERSION 2
 
 enum {
     QF_OK = 0,
@@ -42,6 +42,18 @@ enum {
 };
 
 enum { QF_PREC_F32 = 0, QF_PREC_F64 = 1 };
+
+/* Per-chain outcome of a PCG solve, written on the device to the solvers'
+ * status_out[nrhs] (int).  A chain stops at the first failure with x left at
+ * its last finite iterate.  The reference's preconditioned_cg_fast_test
+ * (hmc_sampler_batch.py:751-846) has no such status: a NaN or a breakdown
+ * there propagates into x, the force and H. */
+enum {
+    QF_SOLVE_CONVERGED = 0, /* relative residual <= rtol */
+    QF_SOLVE_MAX_ITER = 1,  /* max_iter iterations without reaching rtol (the reference's fixed budget) */
+    QF_SOLVE_NONFINITE = 2, /* NaN or Inf in the residual, Re(r^dag z) or Re(p^dag A p): a NaN in phi or b */
+    QF_SOLVE_BREAKDOWN = 3  /* Re(p^dag A p) <= 0: the operator is not positive definite on p */
+};
 
 enum { QF_OP_M = 1, QF_OP_MDAG = 2, QF_OP_B = 3, QF_OP_MDAGM = 4, QF_OP_BPART0 = 16 };
 /* QF_OP_BPART0 + m (m = 0..7): y_tau = Q_m x_tau, the first m factors of the
@@ -122,13 +134,14 @@ int qf_precon_table_from_csr(const long long* crow, const long long* col, const 
  * (hmc_sampler_batch.py:751-846): x0 = 0, fixed max_iter with the per-chain
  * active mask (converged chains freeze; their kernels exit early).  ptab NULL
  * = no preconditioner.  Outputs on the device: err_out[nrhs] (last relative
- * residual) and iters_out[nrhs] (the reference's iteration counter).
+ * residual), iters_out[nrhs] (the reference's iteration counter) and, when
+ * status_out is not NULL, status_out[nrhs] (a QF_SOLVE_* code per chain).
  * Two kernels per iteration; all scalars stay on the device.  work: at least
  * qf_pcg_workspace_bytes(prec, 0, nrhs, L, Ltau) bytes of device memory. */
 size_t qf_pcg_workspace_bytes(int prec, int mixed, int nrhs, int L, int Ltau);
 int qf_pcg_solve(int prec, const void* phi, long long phi_cs, const void* b, void* x, int nrhs, int L, int Ltau, double dtau, double t,
                  const void* ptab, double rtol, int max_iter, void* work, size_t work_bytes, double* err_out, double* iters_out,
-                 void* stream);
+                 int* status_out, void* stream);
 
 /* Mixed-precision PCG (same recurrence and outputs): complex64 vectors and
  * operator with FP64 dot products, and every ru_period iterations a reliable
@@ -138,7 +151,7 @@ int qf_pcg_solve(int prec, const void* phi, long long phi_cs, const void* b, voi
  * preconditioned_cg_fast_test (hmc_sampler_batch.py:751-846). */
 int qf_pcg_mixed_solve(const double* phi64, const float* phi32, long long phi_cs, const void* b, void* x, int nrhs, int L, int Ltau,
                        double dtau, double t, const float* ptab32, double rtol, int max_iter, int ru_period, void* work,
-                       size_t work_bytes, double* err_out, double* iters_out, void* stream);
+                       size_t work_bytes, double* err_out, double* iters_out, int* status_out, void* stream);
 
 /* Ragged batch: nprob independent lattices of different L (one chain each)
  * in the same launches (for example L in {12, 18, 24, 30} with Ltau = 10 L);
@@ -159,7 +172,7 @@ int qf_apply_ragged(int op, int prec, const void* plan_host, const void* plan_de
 size_t qf_pcg_ragged_workspace_bytes(int prec, const void* plan_host);
 int qf_pcg_solve_ragged(int prec, const void* plan_host, const void* plan_dev, const void* phi, const void* b, void* x, double dtau,
                         double t, const void* ptab, double rtol, int max_iter, void* work, size_t work_bytes, double* err_out,
-                        double* iters_out, void* stream);
+                        double* iters_out, int* status_out, void* stream);
 
 /* Fermion force Ft = -dS_f/dphi given xi = (M^dag M)^{-1} psi: the second half of
  * force_f_fast (hmc_sampler_batch.py:2174-2227, per-tau _C.b_vec_per_tau calls and

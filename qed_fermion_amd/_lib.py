@@ -26,6 +26,13 @@ QF_OP_MDAG = 2
 QF_OP_B = 3
 QF_OP_MDAGM = 4
 
+QF_SOLVE_CONVERGED = 0
+QF_SOLVE_MAX_ITER = 1
+QF_SOLVE_NONFINITE = 2
+QF_SOLVE_BREAKDOWN = 3
+SOLVE_STATUS_NAMES = {0: "converged", 1: "max_iter reached", 2: "non-finite residual or step (NaN/Inf in phi or b)",
+                      3: "breakdown (Re(p^dag A p) <= 0)"}
+
 QF_OBS_T1 = 0
 QF_OBS_T21 = 1
 QF_OBS_T2 = 2
@@ -68,13 +75,13 @@ SIGNATURES = {
     "qf_precon_table": (_I, [_I, _P]),
     "qf_precon_table_from_csr": (_I, [_P, _P, _P, _P, _LL, _I, _I, _P]),
     "qf_pcg_workspace_bytes": (_SZ, [_I, _I, _I, _I, _I]),
-    "qf_pcg_solve": (_I, [_I, _P, _LL, _P, _P, _I, _I, _I, _D, _D, _P, _D, _I, _P, _SZ, _P, _P, _P]),
-    "qf_pcg_mixed_solve": (_I, [_P, _P, _LL, _P, _P, _I, _I, _I, _D, _D, _P, _D, _I, _I, _P, _SZ, _P, _P, _P]),
+    "qf_pcg_solve": (_I, [_I, _P, _LL, _P, _P, _I, _I, _I, _D, _D, _P, _D, _I, _P, _SZ, _P, _P, _P, _P]),
+    "qf_pcg_mixed_solve": (_I, [_P, _P, _LL, _P, _P, _I, _I, _I, _D, _D, _P, _D, _I, _I, _P, _SZ, _P, _P, _P, _P]),
     "qf_ragged_plan_bytes": (_SZ, [_I]),
     "qf_ragged_plan": (_I, [_I, _P, _P, _P, _SZ]),
     "qf_apply_ragged": (_I, [_I, _I, _P, _P, _P, _P, _P, _D, _D, _P]),
     "qf_pcg_ragged_workspace_bytes": (_SZ, [_I, _P]),
-    "qf_pcg_solve_ragged": (_I, [_I, _P, _P, _P, _P, _P, _D, _D, _P, _D, _I, _P, _SZ, _P, _P, _P]),
+    "qf_pcg_solve_ragged": (_I, [_I, _P, _P, _P, _P, _P, _D, _D, _P, _D, _I, _P, _SZ, _P, _P, _P, _P]),
     "qf_fermion_force": (_I, [_I, _P, _LL, _P, _P, _I, _I, _I, _D, _D, _P]),
     "qf_boson_force": (_I, [_P, _P, _P, _I, _I, _I, _D, _D, _I, _P]),
     "qf_leapfrog_boson": (_I, [_P, _P, _P, _P, _P, _P, _D, _I, _I, _I, _I, _D, _D, _I, _P]),

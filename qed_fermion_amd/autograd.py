@@ -8,7 +8,9 @@ not kept between forward and backward: backward solves again (the memory-lean
 choice for the largest lattices); otherwise xi is saved.
 
 phi: [nchain, Ltau*L*L*2] float64 (the kernels' layout); psi: [nchain, Ltau*L*L]
-complex128.  precision "fp64" or "mixed" selects the solver.
+complex128.  precision "fp64" or "mixed" selects the solver.  A solve whose
+status is non-finite or a breakdown (kernels.check_status) raises QfError
+instead of returning an action and gradient built on a failed solution.
 """
 import torch
 
@@ -16,11 +18,13 @@ from . import kernels
 
 
 def _solve(phi, psi, L, Ltau, dtau, ptab, rtol, max_iter, precision, ru_period):
+    status = torch.empty(psi.numel() // (L * L * Ltau), dtype=torch.int32, device=psi.device)
     if precision == "mixed":
         xi, _, err = kernels.pcg_mixed_solve(phi, psi, L, Ltau, dtau, ptab=ptab, rtol=rtol, max_iter=max_iter,
-                                             ru_period=ru_period)
+                                             ru_period=ru_period, status=status)
     else:
-        xi, _, err = kernels.pcg_solve(phi, psi, L, Ltau, dtau, ptab=ptab, rtol=rtol, max_iter=max_iter)
+        xi, _, err = kernels.pcg_solve(phi, psi, L, Ltau, dtau, ptab=ptab, rtol=rtol, max_iter=max_iter, status=status)
+    kernels.check_status(status, "FermionAction solve")
     return xi, err
 
 

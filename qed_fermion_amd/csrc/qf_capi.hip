@@ -375,6 +375,7 @@ struct CgWork {
     double* ru_rr;  // mixed only: [2][nrhs]
     double* ru_it;  // mixed only: [2][nrhs]
     int* ru_flag;   // mixed only: [2][nrhs]
+    int* brk;       // [nrhs] breakdown flag of the last update step
     size_t bytes;
 };
 
@@ -399,6 +400,7 @@ CgWork carve(void* base, int prec, int mixed, int nrhs, int L, int Lt, const Til
     w.ru_rr = mixed ? (double*)take(sizeof(double) * 2 * nrhs) : nullptr;
     w.ru_it = mixed ? (double*)take(sizeof(double) * 2 * nrhs) : nullptr;
     w.ru_flag = mixed ? (int*)take(sizeof(int) * 2 * nrhs) : nullptr;
+    w.brk = (int*)take(sizeof(int) * nrhs);
     w.bytes = off;
     return w;
 }
@@ -408,7 +410,7 @@ int pcg_iterate(const R* phi, long long phi_cs, const cplx<R>* b_in, cplx<R>* x,
                 const R* ptab, double rtol, int max_iter, const CgWork& w, const Tiling& t, hipStream_t s,
                 // mixed-precision reliable updates (R = float only)
                 int ru_period, const double* phi64, long long phi64_cs, const double2* b_hi, double2* x_hi, const Hop<double>* hp64,
-                const Tiling* t64, double* err_out, double* iters_out) {
+                const Tiling* t64, double* err_out, double* iters_out, int* status_out) {
     const int Vs = L * L;
     const long long N = (long long)Vs * Lt;
     CguArgs<R> u{};
@@ -429,7 +431,8 @@ int pcg_iterate(const R* phi, long long phi_cs, const cplx<R>* b_in, cplx<R>* x,
 
     // init: x = 0, r = b, z = P b, partials into slot 1, state slot 1
     u.rold = b_in; u.x = x; u.rnew = (cplx<R>*)w.R[1]; u.z = (cplx<R>*)w.Z;
-    u.st_init = w.st + nrhs; u.rz_part = w.rzp + ps; u.rr_part = w.rrp + ps;
+    u.st_init = w.st + nrhs; u.rz_part = w.rzp + ps; u.rr_part = w.rrp + ps; u.brk = w.brk;
+    m.brk = w.brk;
     int rc = launch_cgu<R, CGU_INIT>(u, t, nrhs, s);
     if (rc) return rc;
 
@@ -516,11 +519,11 @@ int pcg_iterate(const R* phi, long long phi_cs, const cplx<R>* b_in, cplx<R>* x,
         if (rc) return rc;
         hipLaunchKernelGGL(cg_finalize_kernel, dim3(nrhs), dim3(64), 0, s, (const double*)w.rp64, t64->nA,
                            (const CgState*)(w.st + last * nrhs), (const double*)w.normb, rtol, 1, err_out, iters_out,
-                           (const Prob*)nullptr);
+                           status_out, (const Prob*)nullptr);
     } else {
         hipLaunchKernelGGL(cg_finalize_kernel, dim3(nrhs), dim3(64), 0, s, (const double*)(w.rrp + last * ps), t.nB,
                            (const CgState*)(w.st + last * nrhs), (const double*)w.normb, rtol, 0, err_out, iters_out,
-                           (const Prob*)nullptr);
+                           status_out, (const Prob*)nullptr);
     }
     QF_LAUNCHED();
     return QF_OK;
@@ -904,7 +907,7 @@ size_t qf_pcg_workspace_bytes(int prec, int mixed, int nrhs, int L, int Ltau) {
 
 int qf_pcg_solve(int prec, const void* phi, long long phi_cs, const void* b, void* x, int nrhs, int L, int Ltau, double dtau, double t,
                  const void* ptab, double rtol, int max_iter, void* work, size_t work_bytes, double* err_out, double* iters_out,
-                 void* stream) {
+                 int* status_out, void* stream) {
     int rc = check_lattice(L, Ltau, nrhs);
     if (rc) return rc;
     if (!phi || !b || !x || !work || !err_out || !iters_out) return fail(QF_ERR_INVALID, "null pointer");
@@ -917,17 +920,17 @@ int qf_pcg_solve(int prec, const void* phi, long long phi_cs, const void* b, voi
     if (prec == QF_PREC_F32)
         return pcg_iterate<float>((const float*)phi, phi_cs, (const float2*)b, (float2*)x, nrhs, L, Ltau, make_hop<float>(dtau, t),
                                   (const float*)ptab, rtol, max_iter, w, tl, s, 0, nullptr, 0, nullptr, nullptr, nullptr, nullptr, err_out,
-                                  iters_out);
+                                  iters_out, status_out);
     if (prec == QF_PREC_F64)
         return pcg_iterate<double>((const double*)phi, phi_cs, (const double2*)b, (double2*)x, nrhs, L, Ltau, make_hop<double>(dtau, t),
                                    (const double*)ptab, rtol, max_iter, w, tl, s, 0, nullptr, 0, nullptr, nullptr, nullptr, nullptr,
-                                   err_out, iters_out);
+                                   err_out, iters_out, status_out);
     return fail(QF_ERR_INVALID, "prec %d", prec);
 }
 
 int qf_pcg_mixed_solve(const double* phi64, const float* phi32, long long phi_cs, const void* b, void* x, int nrhs, int L, int Ltau,
                        double dtau, double t, const float* ptab32, double rtol, int max_iter, int ru_period, void* work,
-                       size_t work_bytes, double* err_out, double* iters_out, void* stream) {
+                       size_t work_bytes, double* err_out, double* iters_out, int* status_out, void* stream) {
     int rc = check_lattice(L, Ltau, nrhs);
     if (rc) return rc;
     if (!phi64 || !phi32 || !b || !x || !work || !err_out || !iters_out) return fail(QF_ERR_INVALID, "null pointer");
@@ -947,7 +950,7 @@ int qf_pcg_mixed_solve(const double* phi64, const float* phi32, long long phi_cs
     const Hop<double> hp64 = make_hop<double>(dtau, t);
     return pcg_iterate<float>(phi32, phi_cs, (const float2*)w.BLO, (float2*)w.XLO, nrhs, L, Ltau, make_hop<float>(dtau, t), ptab32, rtol,
                               max_iter, w, tl, s, ru_period, phi64, phi_cs, (const double2*)b, (double2*)x, &hp64, &t64, err_out,
-                              iters_out);
+                              iters_out, status_out);
 }
 
 // ---------------------------------------------------------------- ragged batch
@@ -1030,12 +1033,12 @@ size_t qf_pcg_ragged_workspace_bytes(int prec, const void* plan_host) {
     if (h->nprob < 1) return 0;
     const size_t vb = al256((size_t)h->nsites * (prec == QF_PREC_F64 ? 16 : 8));
     return 6 * vb + 2 * vb + al256(sizeof(double) * 2 * h->nB_tot) * 2 + al256(sizeof(double) * h->nA_tot) +
-           al256(sizeof(CgState) * 2 * h->nprob) + al256(sizeof(double) * h->nprob);
+           al256(sizeof(CgState) * 2 * h->nprob) + al256(sizeof(double) * h->nprob) + al256(sizeof(int) * h->nprob);
 }
 
 int qf_pcg_solve_ragged(int prec, const void* plan_host, const void* plan_dev, const void* phi, const void* b, void* x, double dtau,
                         double t, const void* ptab, double rtol, int max_iter, void* work, size_t work_bytes, double* err_out,
-                        double* iters_out, void* stream) {
+                        double* iters_out, int* status_out, void* stream) {
     const RaggedHead* h;
     const Prob* pd;
     int rc = ragged_head(plan_host, plan_dev, &h, &pd);
@@ -1061,6 +1064,7 @@ int qf_pcg_solve_ragged(int prec, const void* plan_host, const void* plan_dev, c
         double* pap = (double*)take(sizeof(double) * h->nA_tot);
         CgState* st = (CgState*)take(sizeof(CgState) * 2 * h->nprob);
         double* normb = (double*)take(sizeof(double) * h->nprob);
+        int* brk = (int*)take(sizeof(int) * h->nprob);
         const Hop<R> hp = make_hop<R>(dtau, t);
         const int np = h->nprob;
         const long long ps = h->nB_tot;
@@ -1073,12 +1077,13 @@ int qf_pcg_solve_ragged(int prec, const void* plan_host, const void* plan_dev, c
         CguArgs<R> u{};
         u.ptab = (const R*)ptab; u.prob = pd; u.nprob = np;
         u.rold = (const cplx<R>*)b; u.x = (cplx<R>*)x; u.rnew = (cplx<R>*)Rb[1]; u.z = (cplx<R>*)Z;
-        u.st_init = st + np; u.rz_part = rzp + ps; u.rr_part = rrp + ps;
+        u.st_init = st + np; u.rz_part = rzp + ps; u.rr_part = rrp + ps; u.brk = brk;
         int rc2 = launch_cgu_ragged<R, CGU_INIT>(u, *h, s);
         if (rc2) return rc2;
         MdmArgs<R> m{};
         m.phi = (const R*)phi; m.hop = (const cplx<R>*)HOP; m.hp = hp; m.prob = pd; m.nprob = np; m.nrhs = 1;
         m.normb = normb; m.rtol = rtol; m.dot_part = pap; m.z = (const cplx<R>*)Z; m.y = (cplx<R>*)OP; m.may_stop = 1;
+        m.brk = brk;
         for (int i = 0; i < max_iter; ++i) {
             const int cur = i & 1, prv = cur ^ 1;
             m.iter = i;
@@ -1096,7 +1101,7 @@ int qf_pcg_solve_ragged(int prec, const void* plan_host, const void* plan_dev, c
         }
         const int last = (max_iter - 1) & 1;
         hipLaunchKernelGGL(cg_finalize_kernel, dim3(np), dim3(64), 0, s, (const double*)(rrp + last * ps), 0,
-                           (const CgState*)(st + last * np), (const double*)normb, rtol, 0, err_out, iters_out, pd);
+                           (const CgState*)(st + last * np), (const double*)normb, rtol, 0, err_out, iters_out, status_out, pd);
         QF_LAUNCHED();
         return QF_OK;
     };

@@ -56,6 +56,7 @@ struct CguArgs {
     const cplx<R>* pprev;  // ITER, xmode 2: p' (search direction of the held iteration)
     double* xalpha;        // ITER: [nrhs] alpha of the held iteration (xmode 1 writes, 2 reads)
     const CgState* st_held;  // ITER, xmode 2: state written by step A of the held iteration
+    int* brk;              // [nrhs] breakdown flag of the last ITER step (INIT clears it; optional)
     const Prob* prob;      // ragged batch (nullptr: uniform; grid (nsx, tiles, nrhs))
     int nprob;
 };
@@ -105,8 +106,9 @@ __global__ void __launch_bounds__(TX * TT / SPT) cg_update_kernel(CguArgs<R> A) 
 
     if constexpr (MODE == CGU_ITER) {
         if (A.st[rho].act == 0.0) {   // converged chain: x and r stay frozen (uniform)
-            if (A.xmode == 2 && A.st_held[rho].act != 0.0) {   // stopped right after a held iteration
-                const R ah = (R)A.xalpha[rho];
+            const R ah = (R)A.xalpha[rho];
+            // stopped right after a held iteration (ah = 0: the held step broke down, x stays)
+            if (A.xmode == 2 && A.st_held[rho].act != 0.0 && ah != R(0)) {
 #pragma unroll
                 for (int j = 0; j < SPT; ++j) {
                     const int tau = t0 + sr + j * RT;
@@ -125,8 +127,9 @@ __global__ void __launch_bounds__(TX * TT / SPT) cg_update_kernel(CguArgs<R> A) 
     }
     if constexpr (MODE == CGU_INIT) {
         if (blk == 0 && threadIdx.x == 0) {
-            CgState o; o.rz = 1.0; o.act = 1.0; o.err = 1.0; o.iters = 0.0;
+            CgState o; o.rz = 1.0; o.act = 1.0; o.err = 1.0; o.iters = 0.0; o.stat = 0.0;
             A.st_init[rho] = o;
+            if (A.brk != nullptr) A.brk[rho] = 0;
         }
     }
 
@@ -186,19 +189,30 @@ __global__ void __launch_bounds__(TX * TT / SPT) cg_update_kernel(CguArgs<R> A) 
     }
 
     R alpha = R(0);
+    bool brk = false;   // uniform over the chain: every block reduces the same partials in the same order
     if constexpr (MODE == CGU_ITER) {
         double sums[1];
         const double* src[1] = {A.pap + pin};
         sum_partials_pf<1, 2>(pf, src, nA, sums, red);
-        alpha = (R)(A.st[rho].rz / sums[0]);
+        const double a64 = A.st[rho].rz / sums[0];
+        // breakdown: Re(p^dag A p) <= 0 or a non-finite step.  The step is skipped (x and r
+        // stay, so no NaN reaches them) and step A of the next iteration stops the chain.
+        const bool nonfinite = !isfinite(sums[0]) || (sums[0] > 0.0 && !isfinite(a64));
+        brk = nonfinite || !(sums[0] > 0.0);
+        if (brk && blk == 0 && threadIdx.x == 0 && A.brk != nullptr) A.brk[rho] = nonfinite ? kSolveNonfinite : kSolveBreakdown;
+        alpha = brk ? R(0) : (R)a64;
         if (A.xmode == 1 && blk == 0 && threadIdx.x == 0) A.xalpha[rho] = (double)alpha;
         if (A.xmode == 2) {   // x_{i} = x_{i-1} + alpha' p', kept in registers
             const R ah = (R)A.xalpha[rho];
+            if (ah != R(0)) {
 #pragma unroll
-            for (int j = 0; j < SPT; ++j) xx[j] = mk<R>(xx[j].x + ah * ph[j].x, xx[j].y + ah * ph[j].y);
+                for (int j = 0; j < SPT; ++j) xx[j] = mk<R>(xx[j].x + ah * ph[j].x, xx[j].y + ah * ph[j].y);
+            }
         }
+        if (!brk) {
 #pragma unroll
-        for (int k = 0; k < (PRE ? NLD : SPT); ++k) rw[k] = mk<R>(rw[k].x - alpha * ow[k].x, rw[k].y - alpha * ow[k].y);
+            for (int k = 0; k < (PRE ? NLD : SPT); ++k) rw[k] = mk<R>(rw[k].x - alpha * ow[k].x, rw[k].y - alpha * ow[k].y);
+        }
     }
     if constexpr (PRE) {
 #pragma unroll
@@ -237,7 +251,7 @@ __global__ void __launch_bounds__(TX * TT / SPT) cg_update_kernel(CguArgs<R> A) 
         if (col && tau < Lt) {
             const long long io = vo + (long long)tau * Vs + s;
             if constexpr (MODE == CGU_ITER) {
-                if (A.xmode != 1) A.x[io] = mk<R>(xx[j].x + alpha * pp[j].x, xx[j].y + alpha * pp[j].y);
+                if (A.xmode != 1) A.x[io] = brk ? xx[j] : mk<R>(xx[j].x + alpha * pp[j].x, xx[j].y + alpha * pp[j].y);
             }
             if constexpr (MODE == CGU_INIT) A.x[io] = mk<R>(0, 0);
             if constexpr (MODE == CGU_INIT || MODE == CGU_ITER) A.rnew[io] = r;
@@ -259,9 +273,10 @@ __global__ void __launch_bounds__(TX * TT / SPT) cg_update_kernel(CguArgs<R> A) 
 // Scalars after the last iteration (the reference returns the error of the
 // last iteration and, without graphs, the iteration counter, 828-846).
 // One block per RHS.  If true_res, rr partials hold |b - A x|^2 in FP64 and the
-// error is reported from them even for converged chains.
+// error is reported from them even for converged chains.  status_out (optional)
+// gets the chain's kSolve* code.
 __global__ void cg_finalize_kernel(const double* rr_part, int n, const CgState* st_prev, const double* normb, double rtol,
-                                   int true_res, double* err_out, double* iters_out, const Prob* prob) {
+                                   int true_res, double* err_out, double* iters_out, int* status_out, const Prob* prob) {
     __shared__ double red[64];
     const int rho = blockIdx.x;
     double sums[1];
@@ -287,6 +302,11 @@ __global__ void cg_finalize_kernel(const double* rr_part, int n, const CgState* 
         }
         err_out[rho] = err;
         iters_out[rho] = it;
+        if (status_out != nullptr) {
+            int code = (int)S.stat;
+            if (code == 0) code = !isfinite(err) ? kSolveNonfinite : err <= rtol ? kSolveConverged : kSolveMaxIter;
+            status_out[rho] = code;
+        }
     }
 }
 

@@ -139,6 +139,13 @@ struct CgState {
     double act;    // 1.0 while the chain is still iterating
     double err;    // ||r|| / ||b|| after the last completed iteration
     double iters;  // reference iteration counter
+    double stat;   // 0, or the failure that stopped the chain (kSolveNonfinite, kSolveBreakdown)
 };
+
+// Per-chain solve status (the QF_SOLVE_* codes of the C ABI).
+constexpr int kSolveConverged = 0;   // relative residual <= rtol
+constexpr int kSolveMaxIter = 1;     // max_iter iterations without reaching rtol
+constexpr int kSolveNonfinite = 2;   // NaN or Inf in the residual, Re(r^dag z) or Re(p^dag A p)
+constexpr int kSolveBreakdown = 3;   // Re(p^dag A p) <= 0: A not positive definite on p
 
 }  // namespace qf

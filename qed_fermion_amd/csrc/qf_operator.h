@@ -66,6 +66,7 @@ struct MdmArgs {
     int may_stop;           // 0: the partials are an iterated (not true) residual; do not deactivate
     const int* stop_flag;   // if set: per-chain may_stop (1 right after a reliable update)
     const int* ru_flag;     // MODE 2: skip chains whose reliable-update flag is 0
+    const int* brk;         // MODE 1: per chain, nonzero when the last update step broke down (optional)
     // MODE 2 (true residual in FP64): r = b - M^dag M x written as complex64
     const cplx<R>* b;
     float2* r_lo;
@@ -228,7 +229,7 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
         sum_partials_pf<2, kMdmPartialPrefetch>(pf, src, nBk, sums, red);
         const CgState St = A.st_prev[rho];
         const double nb = (A.iter == 0) ? sqrt(sums[1]) : A.normb[rho];
-        double act, err, rzv, it = St.iters, bt = 0.0;
+        double act, err, rzv, it = St.iters, bt = 0.0, stat = St.stat;
         if (St.act == 0.0) {
             act = 0.0; err = St.err; rzv = St.rz;
         } else {
@@ -236,13 +237,21 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
             const int may = A.stop_flag ? A.stop_flag[rho] : A.may_stop;
             act = (err > A.rtol || !may) ? 1.0 : 0.0;
             rzv = sums[0];
+            // a failed chain stops here: a non-finite residual or r^dag z, or a breakdown of
+            // the last update step (which then left x and r unchanged)
+            const int bk = (A.brk != nullptr && A.iter > 0) ? A.brk[rho] : 0;
+            if (!isfinite(err) || !isfinite(rzv) || !isfinite(nb)) {
+                act = 0.0; stat = (double)kSolveNonfinite;
+            } else if (bk != 0) {
+                act = 0.0; stat = (double)bk;
+            }
             if (A.iter > 0) {
                 bt = sums[0] / St.rz;
                 if (act != 0.0) it += 1.0;
             }
         }
         if (run == 0 && threadIdx.x == 0) {
-            CgState o; o.rz = rzv; o.act = act; o.err = err; o.iters = it;
+            CgState o; o.rz = rzv; o.act = act; o.err = err; o.iters = it; o.stat = stat;
             A.st_out[rho] = o;
             if (A.iter == 0) A.normb[rho] = nb;
         }

@@ -1,7 +1,9 @@
 """Capture-once / replay runners of qed_fermion/fermion_obsr_graph_runner.py, same API.
 
-GEtaGraphRunner (110-191): capture(boson, eta, graph_memory_pool=None, n_warmups=3) -> pool;
-    __call__(boson, eta) -> G_eta
+GEtaGraphRunner (110-191): capture(max_iter_se, graph_memory_pool=None, n_warmups=3) -> pool;
+    __call__(boson, eta) -> G_eta.  capture allocates its own static inputs (boson
+    [1, 2, Lx, Ly, Ltau], eta [Nrv, Ltau*Vs]) and runs the solve with the sampler's
+    max_iter set to max_iter_se while it captures, restoring it afterwards (117-175).
 T1/T21/T2/T4/TvvGraphRunner (365-722): capture(graph_memory_pool=None, n_warmups=3) -> pool;
     __call__(eta, G_eta, boson) -> the estimator's output (Tvv: (Tvv, Tv))
 The estimator graphs hold the HIP field / k-space kernels and the rocFFT
@@ -12,33 +14,47 @@ import torch
 
 class GEtaGraphRunner:
     def __init__(self, se):
-        self.se = se
         self.graph = None
+        self.se = se
         self.input_buffers = {}
         self.output_buffers = {}
+        self.max_iter_se = None
 
-    def capture(self, boson, eta, graph_memory_pool=None, n_warmups=3):
-        ib = {"boson": boson, "eta": eta}
-        s = torch.cuda.Stream()
-        s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(s):
-            for _ in range(n_warmups):
-                self.se.set_eta_G_eta_inner(ib["boson"], ib["eta"])
-            s.synchronize()
-        torch.cuda.current_stream().wait_stream(s)
-        graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(graph, pool=graph_memory_pool):
-            out = self.se.set_eta_G_eta_inner(ib["boson"], ib["eta"])
+    def capture(self, max_iter_se, graph_memory_pool=None, n_warmups=3):
+        """Capture set_eta_G_eta_inner as one graph (fermion_obsr_graph_runner.py:117-175)."""
+        se = self.se
+        hmc = se.hmc_sampler
+        input_buffers = {
+            "boson": torch.zeros((1, 2, se.Lx, se.Ly, se.Ltau), device=se.device, dtype=se.dtype),
+            "eta": torch.zeros((se.Nrv, se.Ltau * se.Vs), device=se.device, dtype=se.cdtype),
+        }
+        max_iter_copy = hmc.max_iter
+        hmc.max_iter = int(max_iter_se)
+        try:
+            s = torch.cuda.Stream()
+            s.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(s):
+                for _ in range(n_warmups):
+                    se.set_eta_G_eta_inner(input_buffers["boson"], input_buffers["eta"])
+                s.synchronize()
+            torch.cuda.current_stream().wait_stream(s)
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph, pool=graph_memory_pool):
+                static_outputs = se.set_eta_G_eta_inner(input_buffers["boson"], input_buffers["eta"])
+        finally:
+            hmc.max_iter = max_iter_copy
         self.graph = graph
-        self.input_buffers = ib
-        self.output_buffers = {"G_eta": out}
+        self.input_buffers = input_buffers
+        self.output_buffers = static_outputs
+        self.max_iter_se = int(max_iter_se)
         return graph.pool()
 
     def __call__(self, boson, eta):
+        """G_eta for boson [1, 2, Lx, Ly, Ltau] and eta [Nrv, Ltau*Vs] (fermion_obsr_graph_runner.py:177-191)."""
         self.input_buffers["boson"].copy_(boson)
         self.input_buffers["eta"].copy_(eta)
         self.graph.replay()
-        return self.output_buffers["G_eta"]
+        return self.output_buffers
 
 
 class _EstimatorGraphRunner:

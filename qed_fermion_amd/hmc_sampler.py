@@ -23,6 +23,7 @@ hmc_sampler_batch.py:20-80) and can be overridden by keyword.
 import collections
 import math
 import os
+import warnings
 
 import torch
 
@@ -105,6 +106,7 @@ class HmcSampler(object):
         # Folder holding check_points/ and preconditioners/ (the reference keeps them next to its
         # sources); None: start from the pi-flux configuration and build P in memory.
         self.ckpt_root = ckpt_root
+        self.last_solve_status = None   # QF_SOLVE_* codes of the last solve(s), on the device
         self.reset()
 
     # ------------------------------------------------------------------ setup
@@ -300,13 +302,20 @@ class HmcSampler(object):
         bs = b.shape[0]
         phi = boson.reshape(bs, -1)
         ptab = self._ptab()
+        status = torch.empty(bs, dtype=torch.int32, device=b.device)
         if self.precision == "mixed":
             x, iters, err = kernels.pcg_mixed_solve(phi.to(torch.float64).contiguous(), b.to(torch.complex128).contiguous(),
                                                     self.Lx, self.Ltau, self.dtau, ptab=ptab, rtol=self.cg_rtol,
-                                                    max_iter=self.max_iter, ru_period=self.ru_period)
+                                                    max_iter=self.max_iter, ru_period=self.ru_period, status=status)
         else:
             x, iters, err = kernels.pcg_solve(self._phi_for(phi, b.dtype), b.contiguous(), self.Lx, self.Ltau, self.dtau,
-                                              ptab=ptab, rtol=self.cg_rtol, max_iter=self.max_iter)
+                                              ptab=ptab, rtol=self.cg_rtol, max_iter=self.max_iter, status=status)
+        self.last_solve_status = status
+        if not self.cuda_graph and not torch.cuda.is_current_stream_capturing():   # readback: outside capture only
+            codes = status.tolist()
+            if max(codes) >= 2:
+                warnings.warn(f"preconditioned_cg_fast_test: chain statuses {codes} (2: non-finite, 3: breakdown); "
+                              "those chains stopped at their last finite iterate", RuntimeWarning)
         iterations = None if self.cuda_graph else iters.to(torch.int64)
         return x, iterations, err.to(self.dtype)
 
@@ -365,6 +374,7 @@ class HmcSampler(object):
         else:
             eng.propose()
         x = self._unperm(eng.x.clone()).to(boson.dtype)
+        self.last_solve_status = eng.status.clone()
         return x, eng.H0.clone(), eng.H1.clone(), eng.iters.mean(dim=0), eng.err.mean(dim=0)
 
     def metropolis_update(self):
@@ -384,10 +394,18 @@ class HmcSampler(object):
             tau_mask[..., t0:t0 + self.tau_block_size] = 1.0
             boson_new, H_old, H_new, cg_converge_iter, cg_r_err = self.leapfrog_proposer5(self.boson, tau_mask)
             accp = torch.rand(self.bs, device=self.device, dtype=torch.float64) < torch.exp(H_old - H_new)
+            # a chain whose solve failed (non-finite or breakdown) is rejected and reported
+            failed = (self.last_solve_status >= 2).any(dim=0)
+            accp &= ~failed
             self.boson[accp] = boson_new[accp]
             thresholds = torch.minimum(torch.exp(H_old - H_new), torch.ones_like(H_old))
+            thresholds = torch.where(failed, torch.zeros_like(thresholds), thresholds)
             for b, th in enumerate(thresholds.tolist()):   # one device-to-host copy per sweep
                 self.threshold_queue[b].append(th)
+            nfail = int(failed.sum().item())
+            if nfail:
+                warnings.warn(f"metropolis_update: {nfail} of {self.bs} chains rejected: a PCG solve was non-finite "
+                              f"or broke down (status {self.last_solve_status.max().item()})", RuntimeWarning)
             self.tau_block_idx += 1
         self.tau_block_idx = 0
         return self.boson, accp, cg_converge_iter, cg_r_err

@@ -59,11 +59,13 @@ class StochaticEstimator:
         self.indices_r2 = torch.combinations(torch.arange(self.Nrv, device=self.device), r=2, with_replacement=False)
 
     def init_cuda_graph(self):
-        """Capture the G_eta solve and the five estimators (stochastic_estimator.py:113-200)."""
+        """Capture the G_eta solve and the five estimators (stochastic_estimator.py:111-200); the
+        solve is captured with the sampler's first CG gear as its budget (134-141)."""
+        hmc = self.hmc_sampler
         if self.cuda_graph_se:
-            dummy_eta = torch.zeros((self.Nrv, self.Ltau * self.Vs), device=self.device, dtype=self.cdtype)
-            dummy_boson = torch.zeros(1, 2, self.Lx, self.Ly, self.Ltau, device=self.device, dtype=self.dtype)
-            self.graph_memory_pool = self.G_eta_graph_runner.capture(dummy_boson, dummy_eta, self.graph_memory_pool)
+            self.graph_memory_pool = self.G_eta_graph_runner.capture(
+                max_iter_se=hmc._MAX_ITERS_TO_CAPTURE[0],
+                graph_memory_pool=self.graph_memory_pool)
             for r in (self.T1_graph_runner, self.T21_graph_runner, self.T2_graph_runner, self.T4_graph_runner,
                       self.Tvv_graph_runner):
                 self.graph_memory_pool = r.capture(self.graph_memory_pool)
@@ -85,7 +87,9 @@ class StochaticEstimator:
         self.G_eta = G_eta
 
     def set_eta_G_eta_inner(self, boson, eta):
-        """psi = M^dag eta; G_eta = (M^dag M)^{-1} psi with one shared field (stochastic_estimator.py:427-433)."""
+        """psi = M^dag eta; G_eta = (M^dag M)^{-1} psi with one shared field (stochastic_estimator.py:427-433).
+        The solve runs the sampler's max_iter iterations at most, as Ot_inv_psi_fast does there
+        (GEtaGraphRunner.capture sets it to max_iter_se while it captures)."""
         hmc = self.hmc_sampler
         phi = boson.permute([0, 4, 3, 2, 1]).reshape(1, -1)
         real = torch.float32 if eta.dtype == torch.complex64 else torch.float64
@@ -95,11 +99,11 @@ class StochaticEstimator:
         ptab = hmc._ptab()
         if hmc.precision == "mixed":
             G_eta, _, _ = kernels.pcg_mixed_solve(phi.double(), psi.to(torch.complex128), self.Lx, self.Ltau, self.dtau,
-                                                  ptab=ptab, rtol=hmc.cg_rtol, max_iter=self.max_iter_se,
+                                                  ptab=ptab, rtol=hmc.cg_rtol, max_iter=hmc.max_iter,
                                                   ru_period=hmc.ru_period)
         else:
             G_eta, _, _ = kernels.pcg_solve(phi, psi, self.Lx, self.Ltau, self.dtau, ptab=ptab, rtol=hmc.cg_rtol,
-                                            max_iter=self.max_iter_se)
+                                            max_iter=hmc.max_iter)
         return G_eta
 
     # ---------------------------------------------------------- bond-bond estimators

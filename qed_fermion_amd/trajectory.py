@@ -71,6 +71,8 @@ class TrajectoryEngine:
         self.parts1 = torch.zeros_like(self.parts0)
         self.err = torch.zeros(n_leap + 1, bs, dtype=f64, device=dev)
         self.iters = torch.zeros_like(self.err)
+        # QF_SOLVE_* code of every solve of the last trajectory, per chain
+        self.status = torch.zeros(n_leap + 1, bs, dtype=torch.int32, device=dev)
         self.accept = torch.zeros(bs, dtype=torch.bool, device=dev)
         self.u = torch.zeros(bs, dtype=f64, device=dev)
         if precon:
@@ -93,13 +95,14 @@ class TrajectoryEngine:
         if self.precision == "mixed":
             kernels.pcg_mixed_solve(self.x, self.psi, L, Lt, self.dtau, ptab=self.ptab, rtol=self.rtol,
                                     max_iter=self.max_iter, ru_period=self.ru_period, phi32=self.x32, x=self.xi,
-                                    work=self.work, err=self.err[k], iters=self.iters[k])
+                                    work=self.work, err=self.err[k], iters=self.iters[k], status=self.status[k])
         elif self.precision == "fp64":
             kernels.pcg_solve(self.x, self.psi, L, Lt, self.dtau, ptab=self.ptab, rtol=self.rtol, max_iter=self.max_iter,
-                              x=self.xi, work=self.work, err=self.err[k], iters=self.iters[k])
+                              x=self.xi, work=self.work, err=self.err[k], iters=self.iters[k], status=self.status[k])
         else:
             kernels.pcg_solve(self.x32, self.psi, L, Lt, self.dtau, ptab=self.ptab, rtol=self.rtol,
-                              max_iter=self.max_iter, x=self.xi, work=self.work, err=self.err[k], iters=self.iters[k])
+                              max_iter=self.max_iter, x=self.xi, work=self.work, err=self.err[k], iters=self.iters[k],
+                              status=self.status[k])
 
     def _force(self):
         if self.precision == "fp32":
@@ -139,10 +142,16 @@ class TrajectoryEngine:
             kernels.momentum_kick(self.p, self.Ff, self.dt, 0.5, L, Lt, tau_mask=self.tau_mask)
         self._ham(self.H1, self.parts1)
 
+    def solve_failed(self):
+        """Per chain: True if a solve of the last trajectory ended non-finite or in a breakdown."""
+        return (self.status >= _lib.QF_SOLVE_NONFINITE).any(dim=0)
+
     def metropolis(self):
-        """accp = rand < exp(H_old - H_new); rejected chains return to x0 (hmc_sampler_batch.py:2910-2918)."""
+        """accp = rand < exp(H_old - H_new); rejected chains return to x0 (hmc_sampler_batch.py:2910-2918).
+        A chain whose solve failed (non-finite or breakdown) is rejected, on the device."""
         self.u.uniform_()
         torch.lt(self.u, torch.exp(self.H0 - self.H1), out=self.accept)
+        self.accept.logical_and_(~self.solve_failed())
         self.x.copy_(torch.where(self.accept.view(-1, 1), self.x, self.x0))
 
     def trajectory(self):

@@ -25,7 +25,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(h, n), n
     assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
-    assert h.qf_version() == 1
+    assert h.qf_version() == 2
 
 
 def test_precon_table_matches_oracle():
@@ -116,3 +116,31 @@ def test_ragged_plan_layout():
     odd = np.array([12, 13], dtype=np.int32)
     assert lib.qf_ragged_plan(2, odd.ctypes.data, Lts.ctypes.data, plan.ctypes.data, nb) == _lib.QF_ERR_ODD_L
     assert lib.qf_ragged_plan(n, Ls.ctypes.data, Lts.ctypes.data, plan.ctypes.data, 8) == _lib.QF_ERR_INVALID
+
+
+def test_buffer_validation_rejects_wrong_buffers():
+    """Every caller-supplied buffer goes through one check (dtype, exact size, contiguity, then
+    device) before its pointer is taken; a wrong buffer raises QfError instead of becoming an
+    out-of-bounds access or a write into a temporary copy."""
+    import torch
+    from qed_fermion_amd import kernels as K
+    from qed_fermion_amd._lib import QF_ERR_INVALID, QfError
+
+    def rejects(fn, what):
+        with pytest.raises(QfError) as e:
+            fn()
+        assert e.value.code == QF_ERR_INVALID and what in str(e.value), str(e.value)
+
+    good = torch.zeros(8, dtype=torch.float64)
+    rejects(lambda: K._buf(None, "x", torch.float64, 8), "required")
+    rejects(lambda: K._buf(good.float(), "x", torch.float64, 8), "dtype")
+    rejects(lambda: K._buf(good[:7], "x", torch.float64, 8), "elements")
+    rejects(lambda: K._buf(torch.zeros(16, dtype=torch.float64)[::2], "x", torch.float64, 8), "contiguous")
+    rejects(lambda: K._buf(good, "x", torch.float64, 8), "not on the GPU")
+    rejects(lambda: K._vecs(torch.zeros(10, dtype=torch.complex64), 4), "multiple")
+    rejects(lambda: K._vecs(torch.zeros(8, dtype=torch.float32), 4), "dtype")
+    rejects(lambda: K._fields64(torch.zeros(2 * 4 * 4 * 3, dtype=torch.float32), 4, 3), "float64")
+    # the public wrappers refuse CPU tensors: there is no CPU fallback
+    rejects(lambda: K.apply("MdagM", torch.zeros(2 * 16 * 3), torch.zeros(16 * 3, dtype=torch.complex64), 4, 3),
+            "GPU")
+    rejects(lambda: K.boson_force(torch.zeros(2 * 16 * 3, dtype=torch.float64), 4, 3, 1.0, 1.0), "GPU")

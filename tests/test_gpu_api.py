@@ -183,9 +183,9 @@ def test_tau_block_updates(dev, cuda_graph):
 def test_stochastic_estimator_solve(dev):
     from qed_fermion_amd.stochastic_estimator import StochaticEstimator
     L, Lt = 4, 30
-    hmc = _sampler(L, Lt, 1, "fp64")
+    hmc = _sampler(L, Lt, 1, "fp64", max_iter=1000)
     hmc.cg_rtol = 1e-12
-    se = StochaticEstimator(hmc, cuda_graph_se=False, Nrv=8, max_iter_se=1000)
+    se = StochaticEstimator(hmc, cuda_graph_se=False, Nrv=8)
     se.initialize()
     eta = se.random_vec_bin()
     e = eta.cpu().numpy()
@@ -198,8 +198,40 @@ def test_stochastic_estimator_solve(dev):
     assert relerr(se.G_eta.cpu().numpy(), G_ref) < 1e-9
     se.cuda_graph_se = True
     se.init_cuda_graph()
+    assert se.G_eta_graph_runner.max_iter_se == hmc._MAX_ITERS_TO_CAPTURE[0] == 1000
     se.set_eta_G_eta(boson, eta)
     assert relerr(se.G_eta.cpu().numpy(), G_ref) < 1e-9
+
+
+def test_geta_graph_runner_reference_call(dev):
+    """GEtaGraphRunner called exactly as stochastic_estimator.py:135-141 does: capture(max_iter_se=...,
+    graph_memory_pool=...) allocates its own static inputs, captures the solve with the sampler's
+    max_iter set to max_iter_se and restores it; replays equal eager solves with that budget."""
+    from qed_fermion_amd.fermion_obsr_graph_runner import GEtaGraphRunner
+    from qed_fermion_amd.stochastic_estimator import StochaticEstimator
+    L, Lt = 4, 30
+    hmc = _sampler(L, Lt, 1, "fp64", max_iter=400)
+    hmc.cg_rtol = 1e-12
+    se = StochaticEstimator(hmc, cuda_graph_se=True, Nrv=6)
+    se.initialize()
+    runner = GEtaGraphRunner(se)
+    pool = runner.capture(max_iter_se=12, graph_memory_pool=None)
+    assert pool is not None and hmc.max_iter == 400
+    assert runner.input_buffers["boson"].shape == (1, 2, L, L, Lt) and runner.input_buffers["eta"].shape == (6, L * L * Lt)
+    for rep in range(3):
+        phi = make_phi(L, Lt, "gauss", seed=30 + rep)
+        boson = _ref_layout(phi)
+        eta = se.random_vec_bin()
+        got = runner(boson, eta).clone()
+        torch.cuda.synchronize()
+        hmc.max_iter = 12                                  # the captured budget, eagerly
+        want = se.set_eta_G_eta_inner(boson, eta)
+        hmc.max_iter = 400
+        torch.cuda.synchronize()
+        assert torch.equal(got, want), rep
+        # 12 iterations leave this field unconverged: the budget was really the captured one
+        full = se.set_eta_G_eta_inner(boson, eta)
+        assert relerr(got.cpu().numpy(), full.cpu().numpy()) > 1e-8
 
 
 def test_z2_noise_orthogonality(dev):
@@ -214,14 +246,46 @@ def test_z2_noise_orthogonality(dev):
     assert off.abs().max().item() < 0.1
 
 
-def test_cg_benchmark_preconditioner_helps(dev):
+@pytest.mark.parametrize("cg_dtype", [torch.complex128, torch.complex64])
+def test_cg_benchmark_matches_oracle(dev, cg_dtype):
+    """CgHmcSampler.benchmark called as cg_hmc_sampler.py:256-341 does: the reference's 9-tuple,
+    fields 6 and 4 only (in that order), and per field the HIP PCG's iteration counter and last
+    residual against oracle.pcg at rtol 1e-7, max_iter 200, with and without the preconditioner."""
     from qed_fermion_amd.cg_hmc_sampler import CgHmcSampler
-    cg = CgHmcSampler(J=1.0, Lx=4, Ltau=40, bs=1, precision="fp32")
-    out = cg.benchmark(bs=3)
-    assert len(out) == 9 and out[0] > 0
-    r = cg.last_benchmark
-    pi_idx = 3                                              # fields: 3 uniform, pi flux, 2 noisy
-    assert r["iters_precon"][pi_idx] < r["iters_plain"][pi_idx]
+    L, Lt, nb = 4, 40, 5
+    cg = CgHmcSampler(J=0.5, Nstep=3000, config=None, Lx=L, Ltau=Lt, bs=1, precision="fp32")
+    cg.cg_dtype = cg_dtype
+    cg.reset()
+    cg.reset_precon()
+    out = cg.benchmark(bs=nb)
+    assert len(out) == 9
+    assert out[0] is None and out[1] == -1.0 and out[2] is None and out[3] == []
+    assert out[4] is None and out[5] is None and out[6] == [] and out[7] == []
+    rec = cg.last_benchmark
+    assert rec["fields"] == [6, 4] and out[8] == rec["err_plain"] and len(out[8]) == 2
+    fields = cg.benchmark_fields(nb).cpu().numpy()
+    assert fields.shape == (2 * nb, 2, L, L, Lt)
+    b = rec["b"].to(torch.complex128).cpu().numpy()
+    tab = O.precon_table(Lt)
+    for k, i in enumerate(rec["fields"]):
+        phi = np.ascontiguousarray(np.transpose(fields[i], (3, 2, 1, 0)))   # [Lt, Ly, Lx, 2]
+        if cg_dtype == torch.complex64:
+            phi = phi.astype(np.float32).astype(np.float64)                  # the field the FP32 kernels see
+        for key_it, key_err, table in (("iters_precon", "err_precon", tab), ("iters_plain", "err_plain", None)):
+            _, it_r, err_r = O.pcg(phi, b, L, Lt, table=table, rtol=1e-7, max_iter=200)
+            it, err = rec[key_it][k], rec[key_err][k]
+            # counters: the same recurrence agrees up to a threshold crossing in FP64, within a few
+            # iterations in complex64; residuals: converged ones below rtol, unconverged ones (the
+            # 200-iteration budget) equal up to the rounding the iteration amplifies
+            slack = 1 if cg_dtype == torch.complex128 else max(3, int(0.05 * it_r))
+            assert abs(it - it_r) <= slack, (i, key_it, it, it_r)
+            if it_r < 200 and it < 200:
+                assert err <= 1e-7 and err_r <= 1e-7, (i, key_err, err, err_r)
+            else:   # complex64 iterates drift from FP64 ones by about 10% in 200 iterations on a rough field
+                band = 0.05 if cg_dtype == torch.complex128 else 0.25
+                assert abs(err - err_r) <= band * err_r, (i, key_err, err, err_r)
+    # field 6 is pi flux plus the narrowest noise, the field family the preconditioner was fitted to
+    assert rec["iters_precon"][0] < rec["iters_plain"][0]
 
 
 def test_autograd_force(dev):
@@ -297,3 +361,33 @@ def test_adaptive_mass_methods_on_device(dev):
     assert p.shape == (2, 2, 4, 4, 40) and p.is_cuda
     ref.sigma_hat = hmc.sigma_hat.cpu()
     assert torch.allclose(hmc.apply_m_inv(p).cpu(), FA.apply_m_inv(ref, p.cpu()), rtol=1e-10, atol=1e-12)
+
+
+def test_wrappers_reject_wrong_buffers(dev):
+    """Output and workspace buffers of the wrong dtype, size or layout raise QfError before any
+    kernel runs (instead of an out-of-bounds write or a write into a temporary copy)."""
+    from qed_fermion_amd import kernels as K
+    L, Lt = 4, 30
+    N = L * L * Lt
+    phi = torch.zeros(1, 2 * N, dtype=torch.float64, device=dev)
+    x = torch.zeros(1, N, dtype=torch.complex128, device=dev)
+    bad_outs = [torch.zeros(1, N - 1, dtype=torch.complex128, device=dev),          # short
+                torch.zeros(1, N, dtype=torch.complex64, device=dev),               # dtype
+                torch.zeros(2, N, dtype=torch.complex128, device=dev)[:, ::2]]      # strided view
+    for out in bad_outs:
+        with pytest.raises(K.QfError):
+            K.apply("MdagM", phi, x, L, Lt, out=out)
+        with pytest.raises(K.QfError):
+            K.fermion_force(phi, x, L, Lt, out=torch.zeros(1, 2 * N - 2, dtype=torch.float64, device=dev))
+    with pytest.raises(K.QfError):
+        K.pcg_solve(phi, x, L, Lt, err=torch.zeros(1, dtype=torch.float32, device=dev))
+    with pytest.raises(K.QfError):
+        K.pcg_solve(phi, x, L, Lt, iters=torch.zeros(2, dtype=torch.float64, device=dev))
+    with pytest.raises(K.QfError):
+        K.pcg_solve(phi, x, L, Lt, work=torch.zeros(16, dtype=torch.uint8, device=dev))
+    with pytest.raises(K.QfError):
+        K.pcg_solve(phi.float(), x, L, Lt)                                          # phi / b dtypes differ
+    with pytest.raises(K.QfError):
+        K.boson_force(phi, L, Lt, 1.0, 1.0, Fp=torch.zeros(2 * N, dtype=torch.float32, device=dev))
+    with pytest.raises(K.QfError):
+        K.hamiltonian(phi, L, Lt, 1.0, 1.0, H=torch.zeros(3, dtype=torch.float64, device=dev))
