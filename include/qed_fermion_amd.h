@@ -143,6 +143,19 @@ int qf_pcg_solve(int prec, const void* phi, long long phi_cs, const void* b, voi
                  const void* ptab, double rtol, int max_iter, void* work, size_t work_bytes, double* err_out, double* iters_out,
                  int* status_out, void* stream);
 
+/* qf_pcg_solve with its workspace in the library's own static device memory instead of a
+ * caller buffer: no counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that forbid hipMalloc);
+ * the call itself allocates nothing.  The scratch is qf_pcg_noalloc_max_bytes() = 64 MiB per
+ * GPU, held by every GPU the library runs on until the process ends; a solve whose
+ * qf_pcg_workspace_bytes(prec, 0, nrhs, L, Ltau) exceeds it is rejected with QF_ERR_INVALID
+ * (one complex128 chain at L=36, Ltau=360 fits).  All calls on one GPU share that GPU's
+ * scratch: calls that overlap on one GPU (other streams, other threads) race. */
+size_t qf_pcg_noalloc_max_bytes(void);
+int qf_pcg_solve_noalloc(int prec, const void* phi, long long phi_cs, const void* b, void* x, int nrhs, int L, int Ltau, double dtau,
+                         double t, const void* ptab, double rtol, int max_iter, double* err_out, double* iters_out, int* status_out,
+                         void* stream);
+
 /* Mixed-precision PCG (same recurrence and outputs): complex64 vectors and
  * operator with FP64 dot products, and every ru_period iterations a reliable
  * update x_hi += x_lo; r = b - (M^dag M)_FP64 x_hi.  b and x are complex128;

@@ -76,6 +76,8 @@ SIGNATURES = {
     "qf_precon_table_from_csr": (_I, [_P, _P, _P, _P, _LL, _I, _I, _P]),
     "qf_pcg_workspace_bytes": (_SZ, [_I, _I, _I, _I, _I]),
     "qf_pcg_solve": (_I, [_I, _P, _LL, _P, _P, _I, _I, _I, _D, _D, _P, _D, _I, _P, _SZ, _P, _P, _P, _P]),
+    "qf_pcg_noalloc_max_bytes": (_SZ, []),
+    "qf_pcg_solve_noalloc": (_I, [_I, _P, _LL, _P, _P, _I, _I, _I, _D, _D, _P, _D, _I, _P, _P, _P, _P]),
     "qf_pcg_mixed_solve": (_I, [_P, _P, _LL, _P, _P, _I, _I, _I, _D, _D, _P, _D, _I, _I, _P, _SZ, _P, _P, _P, _P]),
     "qf_ragged_plan_bytes": (_SZ, [_I]),
     "qf_ragged_plan": (_I, [_I, _P, _P, _P, _SZ]),

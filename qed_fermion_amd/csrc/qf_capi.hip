@@ -621,6 +621,37 @@ template <typename R, typename F> int obs_fused_dispatch(int obs, int L, F&& f) 
 
 }  // namespace
 
+// Workspace of qf_pcg_solve_noalloc: a file-scope device array, so that callers which
+// cannot allocate (code under graph capture, serving loops, harnesses that forbid
+// hipMalloc) still get a solver.  Every GPU the library runs on holds its own copy of
+// these kQfScratchBytes until the process ends.  In host code `scratch` only names the
+// symbol: its device address, per device, comes from hipGetSymbolAddress and is the only
+// thing passed to kernels.
+constexpr size_t kQfScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kQfScratchBytes];
+
+namespace {
+constexpr int kQfMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kQfMaxDevices] = {};
+
+// Device address of the current device's scratch array (looked up once per device).
+int scratch_address(void** out) {
+    int dev = -1;
+    QF_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kQfMaxDevices) return fail(QF_ERR_UNSUPPORTED, "device %d: at most %d devices", dev, kQfMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        QF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (p == nullptr) return fail(QF_ERR_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return QF_OK;
+}
+}  // namespace
+
 // Streaming copy, one 16-byte element per thread (the grid covers the buffer):
 // the achievable-bandwidth reference the operator kernels are compared with.
 __global__ void __launch_bounds__(256) stream_copy_kernel(const float4* __restrict__ a, float4* __restrict__ b, long long n) {
@@ -926,6 +957,27 @@ int qf_pcg_solve(int prec, const void* phi, long long phi_cs, const void* b, voi
                                    (const double*)ptab, rtol, max_iter, w, tl, s, 0, nullptr, 0, nullptr, nullptr, nullptr, nullptr,
                                    err_out, iters_out, status_out);
     return fail(QF_ERR_INVALID, "prec %d", prec);
+}
+
+size_t qf_pcg_noalloc_max_bytes(void) { return kQfScratchBytes; }
+
+int qf_pcg_solve_noalloc(int prec, const void* phi, long long phi_cs, const void* b, void* x, int nrhs, int L, int Ltau, double dtau,
+                         double t, const void* ptab, double rtol, int max_iter, double* err_out, double* iters_out, int* status_out,
+                         void* stream) {
+    int rc = check_lattice(L, Ltau, nrhs);
+    if (rc) return rc;
+    if (prec != QF_PREC_F32 && prec != QF_PREC_F64) return fail(QF_ERR_INVALID, "prec %d", prec);
+    const size_t need = qf_pcg_workspace_bytes(prec, 0, nrhs, L, Ltau);
+    if (need == 0 || need > kQfScratchBytes)
+        return fail(QF_ERR_INVALID, "qf_pcg_solve_noalloc: this solve needs %zu bytes of workspace, more than the %zu bytes of the "
+                                    "library's static scratch (use qf_pcg_solve with a caller workspace)", need, kQfScratchBytes);
+    void* work = nullptr;
+    rc = scratch_address(&work);
+    if (rc) return rc;
+    // the solve writes every part of the workspace before reading it, so the previous call's
+    // contents need no clearing
+    return qf_pcg_solve(prec, phi, phi_cs, b, x, nrhs, L, Ltau, dtau, t, ptab, rtol, max_iter, work, kQfScratchBytes, err_out,
+                        iters_out, status_out, stream);
 }
 
 int qf_pcg_mixed_solve(const double* phi64, const float* phi32, long long phi_cs, const void* b, void* x, int nrhs, int L, int Ltau,
