@@ -84,17 +84,6 @@ int qf_get_tiling(int prec, int nrhs, int L, int Ltau, int* out4);
  * every iteration as hmc_sampler_batch.py:802 does; the results are the same.
  * Returns the previous setting.  No counterpart in the reference. */
 int qf_set_paired_x_updates(int on);
-/* dst = src (bytes a multiple of 16, 16-byte aligned), one 16-byte element
- * per thread: the streaming-bandwidth reference that bench.py reports beside
- * the operator's roofline.  No counterpart in the reference. */
-int qf_stream_copy(const void* src, void* dst, long long bytes, void* stream);
-/* dst = src in chunks of chunk_bytes (both multiples of 16, 16-byte aligned):
- * one work-group per chunk copies it front to back, so that the copy runs as
- * many concurrent address streams as the tau-marching operator does.  The
- * bandwidth reference for that access pattern.  No counterpart in the
- * reference. */
-int qf_stream_copy_chunks(const void* src, void* dst, long long bytes, long long chunk_bytes, void* stream);
-
 /* y = op(phi) x for nrhs right-hand sides.  op = QF_OP_M: get_M_sparse's M
  * (hmc_sampler_batch.py:1884-1906); QF_OP_MDAG: M^dag (_C.mh_vec, :2406);
  * QF_OP_MDAGM: M^dag M (_C.mhm_vec, :772, :799); QF_OP_B: y_tau = B_tau x_tau

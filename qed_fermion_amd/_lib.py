@@ -63,8 +63,6 @@ SIGNATURES = {
     "qf_set_tuning_shape": (_I, [_I, _I, _I, _I, _I, _I, _I]),
     "qf_clear_tuning": (_I, []),
     "qf_set_paired_x_updates": (_I, [_I]),
-    "qf_stream_copy": (_I, [_P, _P, _LL, _P]),
-    "qf_stream_copy_chunks": (_I, [_P, _P, _LL, _LL, _P]),
     "qf_get_tiling": (_I, [_I, _I, _I, _I, _P]),
     "qf_apply": (_I, [_I, _I, _P, _LL, _P, _P, _I, _I, _I, _D, _D, _P]),
     "qf_hop_table": (_I, [_I, _P, _LL, _I, _I, _I, _D, _D, _P, _P]),

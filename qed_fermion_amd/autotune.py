@@ -92,9 +92,10 @@ def _dtypes(prec):
     return (torch.float32, torch.complex64) if prec == QF_PREC_F32 else (torch.float64, torch.complex128)
 
 
-def tune_operator(prec, L, Lt, nrhs, shared_field=True, reps=10, seed=0):
-    """Best (by, t) for M^dag M on nrhs right-hand sides (from the hop table;
-    one field shared by all RHS if shared_field, else one per RHS).  Returns
+def tune_operator(prec, L, Lt, nrhs, shared_field=True, reps=10, seed=0, source="hop"):
+    """Best (by, t) for M^dag M on nrhs right-hand sides, applied from the hop table
+    (source "hop") or from phi (source "phi", the coefficients' sincos in the kernel);
+    one field shared by all RHS if shared_field, else one per RHS.  Returns
     (by, t, ms_per_launch, table)."""
     rd, cd = _dtypes(prec)
     g = torch.Generator(device="cpu").manual_seed(seed)
@@ -103,14 +104,19 @@ def tune_operator(prec, L, Lt, nrhs, shared_field=True, reps=10, seed=0):
     phi = ((torch.rand(nchain, 2 * N, generator=g, dtype=torch.float64) * 2 - 1) * math.pi).to("cuda", rd)
     x = torch.randn(nrhs, N, generator=g, dtype=torch.complex128).to("cuda", cd)
     y = torch.empty_like(x)
-    hop = kernels.hop_table(phi, L, Lt)
+    if source == "hop":
+        hop = kernels.hop_table(phi, L, Lt)
+        run = lambda: kernels.apply_hop("MdagM", hop, x, L, Lt, out=y)
+    elif source == "phi":
+        run = lambda: kernels.apply("MdagM", phi, x, L, Lt, out=y)
+    else:
+        raise ValueError(f"source {source!r}")
     table = {}
     for by in valid_by(L, prec):
         for t in _t_candidates(Lt):
             kernels.set_tuning_shape(prec, L, Lt, nrhs, by, t, 0)
-            table[(by, t)] = _time(lambda: kernels.apply_hop("MdagM", hop, x, L, Lt, out=y), reps)
-    _refine(table, lambda k: kernels.set_tuning_shape(prec, L, Lt, nrhs, k[0], k[1], 0),
-            lambda: kernels.apply_hop("MdagM", hop, x, L, Lt, out=y), reps)
+            table[(by, t)] = _time(run, reps)
+    _refine(table, lambda k: kernels.set_tuning_shape(prec, L, Lt, nrhs, k[0], k[1], 0), run, reps)
     kernels.set_tuning_shape(prec, L, Lt, nrhs, 0, 0, 0)
     (by, t), ms = min(table.items(), key=lambda kv: kv[1])
     return by, t, ms, table
@@ -169,12 +175,15 @@ def tune(prec, L, Lt, nrhs, kind="pcg", persist=True, force=False, **kw):
 
     kind = "pcg": the PCG iteration with one field per RHS (the sampler's
     solves); kind = "operator": M^dag M with one field shared by the RHS (the
-    stochastic estimator's solves).  A cached result for this device is
+    stochastic estimator's solves; shared_field=False and source="phi" time it
+    with one field per RHS from phi, as PCG step A applies it).  A cached result for this device is
     reused unless force.  Returns {"by", "t", "tb", "ms"}."""
     path = cache_path()
     cache = _load_cache(path) if persist else {}
     dk = device_key()
     key = _shape_key(prec, L, Lt, nrhs, kind)
+    if kind == "operator" and (not kw.get("shared_field", True) or kw.get("source", "hop") != "hop"):
+        key += f":{'shared' if kw.get('shared_field', True) else 'per_chain'}:{kw.get('source', 'hop')}"
     hit = cache.get(dk, {}).get(key)
     if hit is None or force:
         if kind == "pcg":

@@ -652,64 +652,9 @@ int scratch_address(void** out) {
 }
 }  // namespace
 
-// Streaming copy, one 16-byte element per thread (the grid covers the buffer):
-// the achievable-bandwidth reference the operator kernels are compared with.
-__global__ void __launch_bounds__(256) stream_copy_kernel(const float4* __restrict__ a, float4* __restrict__ b, long long n) {
-    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) b[i] = a[i];
-}
-
-// Chunked copy: work-group k copies elements [k*chunk, (k+1)*chunk) front to back, four 16-byte
-// elements per thread in flight per step.  The work-groups in flight read and write as many
-// separate address streams as a tau-marching operator launch does (one per work-group and
-// array), where stream_copy_kernel has one contiguous front.
-__global__ void __launch_bounds__(256) chunk_copy_kernel(const float4* __restrict__ a, float4* __restrict__ b,
-                                                         long long n, long long chunk) {
-    const long long lo = (long long)blockIdx.x * chunk;
-    const long long hi = lo + chunk < n ? lo + chunk : n;
-    for (long long i = lo + threadIdx.x; i < hi; i += 4 * 256) {
-        float4 v[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (i + 256 * u < hi) v[u] = a[i + 256 * u];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (i + 256 * u < hi) b[i + 256 * u] = v[u];
-    }
-}
-
 extern "C" {
 
 int qf_version(void) { return QF_ABI_VERSION; }
-
-int qf_stream_copy(const void* src, void* dst, long long bytes, void* stream) {
-    if (!src || !dst) return fail(QF_ERR_INVALID, "null pointer");
-    if (bytes < 0 || bytes % 16 != 0 || ((uintptr_t)src | (uintptr_t)dst) % 16 != 0)
-        return fail(QF_ERR_INVALID, "copy of %lld bytes: size and addresses must be multiples of 16", bytes);
-    const long long n = bytes / 16;
-    if ((n + 255) / 256 > 0x7fffffffLL) return fail(QF_ERR_INVALID, "copy of %lld bytes is too large", bytes);
-    if (n == 0) return QF_OK;
-    hipLaunchKernelGGL(stream_copy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
-                       (const float4*)src, (float4*)dst, n);
-    QF_LAUNCHED();
-    return QF_OK;
-}
-
-int qf_stream_copy_chunks(const void* src, void* dst, long long bytes, long long chunk_bytes, void* stream) {
-    if (!src || !dst) return fail(QF_ERR_INVALID, "null pointer");
-    if (bytes < 0 || bytes % 16 != 0 || chunk_bytes <= 0 || chunk_bytes % 16 != 0 ||
-        ((uintptr_t)src | (uintptr_t)dst) % 16 != 0)
-        return fail(QF_ERR_INVALID, "copy of %lld bytes in chunks of %lld: sizes and addresses must be multiples of 16",
-                    bytes, chunk_bytes);
-    const long long n = bytes / 16, chunk = chunk_bytes / 16;
-    const long long nchunks = (n + chunk - 1) / chunk;
-    if (nchunks > 0x7fffffffLL) return fail(QF_ERR_INVALID, "copy of %lld bytes: too many chunks", bytes);
-    if (n == 0) return QF_OK;
-    hipLaunchKernelGGL(chunk_copy_kernel, dim3((unsigned)nchunks), dim3(256), 0, (hipStream_t)stream,
-                       (const float4*)src, (float4*)dst, n, chunk);
-    QF_LAUNCHED();
-    return QF_OK;
-}
 
 const char* qf_last_error(void) { return g_err.c_str(); }
 

@@ -183,17 +183,29 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
             return;
         }
     }
-    // Pipelined coefficients from phi (PHIPF): the angles of slice tau+2 are loaded in step
-    // tau and their sincos is taken at the start of step tau+1, so that no step waits for a
-    // load it has issued itself.  From the hop table (HOPT) the coefficients are loaded
-    // straight into registers one or two steps ahead.
+    // Pipelined coefficients from phi (PHIPF): the angles and x of the slices a step will need
+    // are loaded PFD steps before it (the angles of slice tau+1+PFD and x_{tau+1+PFD} in step
+    // tau), into a ring of PFD register sets, and the sincos is taken when the angles are
+    // used, so that no step waits for a load it has issued itself.  PFD = 1: a second step of
+    // lookahead (PFD = 2) took the 2 x 2 patch from 125 to 141 VGPRs, four to three waves per
+    // SIMD, and the per-chain M^dag M from 0.58 to 0.44 of the HBM peak: the tau march is held
+    // by the latency of its exchange barriers, which other resident waves hide, more than by
+    // memory latency.  From the hop table (HOPT) the coefficients are loaded straight into
+    // registers one or two steps ahead.
     constexpr bool PHIPF = PIPE && !HOPT;
+    constexpr int PFD = 1;
     constexpr bool REGPF = PIPE && HOPT && mdm_regpf<R, BY, MODE>();
     constexpr bool TOUCH = !PHIPF && !REGPF;
     // coefficient source of one slice: hop entries or phi
     const unsigned cslice = HOPT ? 2u * sstride : 2u * (unsigned)Vs * (unsigned)sizeof(R), nlanes = blockDim.x;
     const __amdgpu_buffer_rsrc_t rs_c = HOPT ? rs_h : buf_rsrc(phi, cslice * (unsigned)Lt);
     auto ldphi = [&](cplx<R> (&q)[2 * BY], int tau) { load_patch_buf<R, BY>(q, rs_c, po, (unsigned)tau * cslice, L); };
+    // Slices loaded ahead, clamped to the run: angles to its last slice, x to its halo slice
+    // (loads past the end of a run then read a slice just read, an L2 hit, and are unused),
+    // and wrapped cyclically.
+    auto wrap = [&](int tau) { return tau >= Lt ? tau - Lt : tau; };
+    auto clampA = [&](int tau) { return wrap(min(tau, t1 - 1)); };
+    auto clampX = [&](int tau) { return wrap(min(tau, t1)); };
 
     // prologue loads first: they do not depend on the CG scalars (the partial sums
     // those come from are requested before them)
@@ -206,7 +218,9 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
     const int t0n = (t0 + 1 == Lt) ? 0 : t0 + 1;
     CoefC<R, BY> c0, c1, c2;
     Raw rw, rx;
-    cplx<R> qa[PHIPF ? 2 * BY : 1], qb[PHIPF ? 2 * BY : 1], qn[PHIPF ? 2 * BY : 1];   // angles of tm, t0, t0+1
+    // PHIPF: angles of tm and t0, and the ring qr[k] of the angles of t0+1+k
+    cplx<R> qa[PHIPF ? 2 * BY : 1], qb[PHIPF ? 2 * BY : 1], qr[PFD][PHIPF ? 2 * BY : 1];
+    Raw xr[PFD];   // PHIPF: ring of x slices loaded ahead
     if constexpr (PHIPF) {
         ldphi(qa, tm);
         ldphi(qb, t0);
@@ -217,7 +231,8 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
     ldraw(rw, tm);
     ldraw(rx, t0);
     if constexpr (PHIPF) {
-        if (t0 + 1 < t1) ldphi(qn, t0n);
+#pragma unroll
+        for (int k = 0; k < PFD; ++k) ldphi(qr[k], clampA(t0 + 1 + k));
     }
 
     if constexpr (MODE == MDM_CG) {
@@ -371,17 +386,20 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
         touch_lines<NTOUCH>(td, rs_c, (unsigned)t0n * cslice, cslice, nlanes);
     }
     ldraw(rx, t0n);   // x_{t0+1}
+    if constexpr (PHIPF && PFD == 2) ldraw(xr[0], clampX(t0 + 2));
     apply_B2c<R, BY>(U, c2, W, c0, lds, xa, phase);
     axpy_patch<R, BY>(Y, X, sgnM(t0) * cn, U);
     make(X, rx);
     axpy_patch<R, BY>(Yn, X, sgnM(t0n) * cn, W);
 
     if constexpr (PHIPF) {
-    // one step; cA = C_tau, cB receives C_{tau+1} from the angles qn loaded one step earlier
-    auto stepq = [&](int tau, CoefC<R, BY>& cA, CoefC<R, BY>& cB) {
+    // one step; cA = C_tau, cB receives C_{tau+1} from the angles loaded PFD steps earlier.
+    // K is the step's parity in the two-step unrolled loop: ring slot K % PFD holds the angles
+    // of slice tau+1 and x_{tau+2}; slot (K+1) % PFD receives x_{tau+1+PFD}.
+    auto stepq = [&](auto K, int tau, CoefC<R, BY>& cA, CoefC<R, BY>& cB) {
+        constexpr int kc = decltype(K)::value % PFD, kn = (decltype(K)::value + 1) % PFD;
         const bool more = tau + 1 < t1;
-        const int tn = (tau + 1 == Lt) ? 0 : tau + 1;
-        const int tnn = (tn + 1 == Lt) ? 0 : tn + 1;
+        const int tnn = wrap(tau + 2);
         cplx<R> bb[2 * BY];
 #pragma unroll
         for (int k = 0; k < 2 * BY; ++k) W[k] = Yn[k];
@@ -389,14 +407,12 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
 #pragma unroll
             for (int k = 0; k < 2 * BY; ++k) U[k] = X[k];
             if constexpr (MODE == MDM_CG) {
-                if (g.valid) store_patch_g<R, BY>(X, pnew + (long long)tn * Vs, po, L);
+                if (g.valid) store_patch_g<R, BY>(X, pnew + (long long)wrap(tau + 1) * Vs, po, L);
             }
-            coefc_from_angles<R, BY>(cB, qn, A.hp);           // C_{tau+1}
-            // angles of slice tau+2; in the last step of the run they are not used, and slice
-            // tau+1 (just read, an L2 hit) is loaded instead so that the load is unconditional
-            ldphi(qn, tau + 2 < t1 ? tnn : tn);
-            ldraw(rx, tnn);                                   // x_{tau+2} (the halo slice at the end of the run)
-            apply_B2c<R, BY>(W, cA, U, cB, lds, xa, phase);   // W = B_tau y_{tau+1}, U = B_{tau+1} x_{tau+1}
+            coefc_from_angles<R, BY>(cB, qr[kc], A.hp);        // C_{tau+1}
+            ldphi(qr[kc], clampA(tau + 1 + PFD));              // angles of slice tau+1+PFD
+            ldraw(xr[kn], clampX(tau + 1 + PFD));              // x_{tau+1+PFD} (the halo slice at the end of the run)
+            apply_B2c<R, BY>(W, cA, U, cB, lds, xa, phase);    // W = B_tau y_{tau+1}, U = B_{tau+1} x_{tau+1}
         } else {
             apply_Bc<R, BY>(W, cA, lds, xa, phase);
         }
@@ -404,14 +420,14 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
         if (more) {
 #pragma unroll
             for (int k = 0; k < 2 * BY; ++k) Y[k] = Yn[k];
-            make(X, rx);                                      // x_{tau+2}
-            axpy_patch<R, BY>(Yn, X, sgnM(tnn) * cn, U);      // y_{tau+2}
+            make(X, xr[kc]);                                   // x_{tau+2}
+            axpy_patch<R, BY>(Yn, X, sgnM(tnn) * cn, U);       // y_{tau+2}
         }
     };
     for (int tau = t0;;) {
-        stepq(tau, c0, c1);
+        stepq(std::integral_constant<int, 0>{}, tau, c0, c1);
         if (++tau >= t1) break;
-        stepq(tau, c1, c0);
+        stepq(std::integral_constant<int, 1>{}, tau, c1, c0);
         if (++tau >= t1) break;
     }
     } else {

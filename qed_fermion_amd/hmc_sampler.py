@@ -28,7 +28,7 @@ import warnings
 import torch
 
 from . import checkpoint, fourier_accel, kernels, precon
-from .trajectory import TrajectoryEngine, couplings
+from .trajectory import TrajectoryEngine, adjust_delta_t, couplings
 
 
 def _env(name, default, cast):
@@ -412,18 +412,7 @@ class HmcSampler(object):
 
     def adjust_delta_t(self):
         """Per-chain step size x0.9 / x1.1 outside the acceptance window (hmc_sampler_batch.py:699-724)."""
-        for b in range(self.bs):
-            q = self.threshold_queue[b]
-            if len(q) < q.maxlen:
-                continue
-            avg = sum(q) / len(q)
-            if avg < self.lower_limit:
-                self.delta_t_tensor[b] *= 0.9
-            elif self.upper_limit < avg:
-                self.delta_t_tensor[b] *= 1.1
-            else:
-                continue
-            q.clear()
+        adjust_delta_t(self.delta_t_tensor, self.threshold_queue, self.lower_limit, self.upper_limit)
 
     # ------------------------------------------------------------------ graphs
     def initialize_force_graph(self):

@@ -101,28 +101,6 @@ def apply(op, phi, x, L, Ltau, dtau=0.1, t=1.0, out=None):
     return y.view(x.shape)
 
 
-def stream_copy(src, dst):
-    """dst = src through qf_stream_copy (one 16-byte element per thread): the streaming
-    bandwidth reference for the operator kernels."""
-    _check_dev(src, dst)
-    nbytes = src.numel() * src.element_size()
-    if dst.numel() * dst.element_size() != nbytes:
-        raise QfError(_lib.QF_ERR_INVALID, "stream_copy: buffers differ in size")
-    call("qf_stream_copy", ptr(src), ptr(dst), nbytes, stream_handle())
-    return dst
-
-
-def stream_copy_chunks(src, dst, chunk_bytes):
-    """dst = src through qf_stream_copy_chunks: one work-group per chunk of chunk_bytes, copied
-    front to back (as many concurrent address streams as a tau-marching operator launch)."""
-    _check_dev(src, dst)
-    nbytes = src.numel() * src.element_size()
-    if dst.numel() * dst.element_size() != nbytes:
-        raise QfError(_lib.QF_ERR_INVALID, "stream_copy_chunks: buffers differ in size")
-    call("qf_stream_copy_chunks", ptr(src), ptr(dst), nbytes, int(chunk_bytes), stream_handle())
-    return dst
-
-
 def hop_table(phi, L, Ltau, dtau=0.1, t=1.0, out=None):
     """Per-link factor coefficients tanh(a_g) e^{i phi} of each field in phi [nchain, 2N]."""
     N = L * L * Ltau

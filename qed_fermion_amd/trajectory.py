@@ -30,6 +30,28 @@ import torch
 from . import _lib, kernels
 
 
+def adjust_delta_t(delta_t, queues, lower_limit=0.25, upper_limit=0.5):
+    """Per-chain step-size control of hmc_sampler_batch.py:699-724, in place on ``delta_t``
+    (a [bs] tensor, any device): a chain whose deque of clipped thresholds min(1, e^{-dH}) is
+    full and averages below lower_limit gets dt x 0.9, above upper_limit dt x 1.1; its deque is
+    then cleared.  Chains with a partial deque, or an average inside the window, are left alone.
+    Returns the list of chains adjusted."""
+    changed = []
+    for b, q in enumerate(queues):
+        if len(q) < q.maxlen:
+            continue
+        avg = sum(q) / len(q)
+        if avg < lower_limit:
+            delta_t[b] *= 0.9
+        elif upper_limit < avg:
+            delta_t[b] *= 1.1
+        else:
+            continue
+        q.clear()
+        changed.append(b)
+    return changed
+
+
 def couplings(J, K, dtau, Nf=2):
     """(cJ, cK) = (1/(J' dtau^2), K') with J' = J/dtau*Nf/4, K' = K*dtau*Nf/2 (hmc_sampler_batch.py:173-174, 1344)."""
     Jp = J / dtau * Nf / 4
@@ -186,4 +208,5 @@ class TrajectoryEngine:
         self.graph.replay()
 
     def acceptance(self):
+        """Clipped Metropolis threshold min(1, e^{H0 - H1}) per chain (hmc_sampler_batch.py:2921)."""
         return torch.minimum(torch.exp(self.H0 - self.H1), torch.ones_like(self.H0))

@@ -22,9 +22,17 @@ def test_traffic_follows_the_tiling():
         assert got == e["traffic_bytes"]
         # at least the algorithmic bytes; the excess is the tau halo, which shrinks with the run length
         assert e["algorithmic_bytes"] <= e["traffic_bytes"] < 1.25 * e["algorithmic_bytes"]
-    per_chain = sorted((e["tiling"]["t"], e["traffic_bytes"]) for e in rec["entries"]
-                       if e["kind"] == "per_chain" and e["tiling"]["by"] == 4)
-    assert [b for _, b in per_chain] == sorted((b for _, b in per_chain), reverse=True)
+    for kind in ("per_chain", "per_chain_phi"):
+        for by in (2, 4):
+            runs = sorted((e["tiling"]["t"], e["traffic_bytes"]) for e in rec["entries"]
+                          if e["kind"] == kind and e["tiling"]["by"] == by)
+            assert [b for _, b in runs] == sorted((b for _, b in runs), reverse=True)   # longer runs, less halo
+    # from phi: (24 + 8 * 3 / t) bytes per site per chain, the tau halo of a t-slice run
+    for e in rec["entries"]:
+        if e["kind"] == "per_chain_phi":
+            t = e["tiling"]["t"]
+            assert abs(e["traffic_over_algorithmic"] - (24 + 24 / t) / 24) < 0.01, e["tiling"]
+    assert bench.traffic_of(36, 360, 32, "per_chain_phi", {"by": 2, "t": 24}) is not None
     assert bench.traffic_of(36, 360, 32, "per_chain", {"by": 6, "t": 7}) is None      # not measured
     assert bench.traffic_of(36, 360, 31, "per_chain", {"by": 4, "t": 12}) is None     # other shape
 

@@ -74,17 +74,10 @@ def test_argument_validation_without_gpu():
     out = (ctypes.c_int * 4)()
     assert h.qf_get_tiling(0, 1, 36, 360, out) == 0 and out[0] in (2, 4, 6)
     assert h.qf_get_tiling(0, 1, 66, 660, out) == 0 and out[0] == 6
-    # the streaming copy validates before it launches anything
-    buf = (ctypes.c_char * 64)()
-    assert h.qf_stream_copy(None, None, 16, None) == _lib.QF_ERR_INVALID
-    assert h.qf_stream_copy(ctypes.addressof(buf), ctypes.addressof(buf), 12, None) == _lib.QF_ERR_INVALID
-    assert "multiples of 16" in h.qf_last_error().decode()
-    assert h.qf_stream_copy(ctypes.addressof(buf), ctypes.addressof(buf), 0, None) == _lib.QF_OK
-    assert h.qf_stream_copy_chunks(None, None, 16, 16, None) == _lib.QF_ERR_INVALID
-    assert h.qf_stream_copy_chunks(ctypes.addressof(buf), ctypes.addressof(buf), 32, 0, None) == _lib.QF_ERR_INVALID
-    assert h.qf_stream_copy_chunks(ctypes.addressof(buf), ctypes.addressof(buf), 32, 24, None) == _lib.QF_ERR_INVALID
-    assert "multiples of 16" in h.qf_last_error().decode()
-    assert h.qf_stream_copy_chunks(ctypes.addressof(buf), ctypes.addressof(buf), 0, 16, None) == _lib.QF_OK
+    # the allocation-free solver refuses a solve larger than its static scratch before any launch
+    assert h.qf_pcg_noalloc_max_bytes() == 64 << 20
+    rc = h.qf_pcg_solve_noalloc(1, None, 0, None, None, 64, 36, 360, 0.1, 1.0, None, 1e-9, 10, None, None, None, None)
+    assert rc == _lib.QF_ERR_INVALID and "static scratch" in h.qf_last_error().decode()
 
 
 def test_paired_x_switch():

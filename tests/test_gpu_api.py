@@ -317,29 +317,6 @@ def test_autograd_force(dev):
     assert torch.equal(ph2.grad, ph.grad)
 
 
-def test_stream_copy(dev):
-    """The streaming-bandwidth reference copies exactly and rejects sizes that are not whole
-    16-byte elements."""
-    from qed_fermion_amd import kernels as K
-    a = torch.randn(4100, device=dev)   # 16400 bytes, 1025 elements: a partial last work-group
-    b = torch.zeros_like(a)
-    K.stream_copy(a, b)
-    torch.cuda.synchronize()
-    assert torch.equal(a, b)
-    c, d = torch.zeros(3, device=dev), torch.zeros(3, device=dev)
-    with pytest.raises(K.QfError):
-        K.stream_copy(c, d)
-    # chunked: chunks that split the buffer unevenly (a shorter last chunk), chunks longer than
-    # one step of a work-group (several steps, a partial last step), and one chunk for all
-    for chunk in (16, 48, 4096 * 16 + 32, 16400, 1 << 20):
-        b.zero_()
-        K.stream_copy_chunks(a, b, chunk)
-        torch.cuda.synchronize()
-        assert torch.equal(a, b), chunk
-    with pytest.raises(K.QfError):
-        K.stream_copy_chunks(a, b, 24)
-
-
 def test_adaptive_mass_methods_on_device(dev):
     """The sampler's adaptive-mass methods (hmc_sampler_batch.py:1133-1277) on the device equal the
     same utilities on CPU tensors."""
