@@ -12,9 +12,11 @@ hmc_sampler_batch.py:1026-1043) and are thermalised by --therm untimed trajector
 evolve them (accept/reject) while the reference's per-chain step-size control
 (adjust_delta_t, hmc_sampler_batch.py:699-724) brings the acceptance into its window.
 The timed trajectories continue the same chains; nothing is reset between them.  The PCG
-iteration cap is the reference's gear mechanism (_MAX_ITERS_TO_CAPTURE,
-hmc_sampler_batch.py:305-309): chosen after the first thermalisation trajectories as a
-margin over the iterations the solves take, and raised if a solve reaches it.
+iteration caps are the reference's gear mechanism (_MAX_ITERS_TO_CAPTURE,
+hmc_sampler_batch.py:305-309), one per solve of the trajectory: after the first
+thermalisation trajectories each solve's cap is a margin over the iterations it takes,
+raised if a solve reaches it (a converged batch still runs its graph to the cap, at about
+28 us per empty iteration of dispatch latency).
 
 Reported next to it:
   * the same GPU's FP64 plain path: complex128 PCG, at the same batch and with one chain,
@@ -66,7 +68,8 @@ def parse():
     ap.add_argument("--therm", type=int, default=100, help="untimed thermalisation trajectories (dt adapted)")
     ap.add_argument("--delta-t", type=float, default=0.06, help="initial leapfrog step (the reference's delta_t)")
     ap.add_argument("--max-iter", type=int, default=1000, help="PCG iteration cap before the gear is chosen")
-    ap.add_argument("--gear-margin", type=float, default=1.25, help="cap = margin x the largest iteration count seen")
+    ap.add_argument("--gear-margin", type=float, default=1.1,
+                    help="each solve's cap = margin x the largest iteration count it took during thermalisation")
     ap.add_argument("--rtol", type=float, default=1e-10, help="PCG tolerance of the headline solves")
     ap.add_argument("--ref-max-iter", type=int, default=400,
                     help="the reference's fixed PCG budget (max_iter), used on the uniform field and the extras")
@@ -256,6 +259,7 @@ class Chains:
         self.capture()
 
     def capture(self):
+        """(Re)capture the trajectory graph with the current caps, keeping the chains' state."""
         x, dt = self.eng.x.clone(), self.eng.dt.clone()
         self.eng.graph = None
         t0 = time.time()
@@ -263,39 +267,44 @@ class Chains:
         self.torch.cuda.synchronize()
         self.eng.x.copy_(x)
         self.eng.dt.copy_(dt)
-        log(f"[bench] {self.tag}: captured trajectory graph (cap {self.eng.max_iter}) in {time.time() - t0:.1f}s")
+        log(f"[bench] {self.tag}: captured trajectory graph (caps {self.eng.max_iters}) in {time.time() - t0:.1f}s")
 
-    def set_cap(self, cap):
-        if cap != self.eng.max_iter:
-            self.eng.max_iter = cap
+    def set_caps(self, caps):
+        if list(caps) != self.eng.max_iters:
+            self.eng.max_iter = list(caps)
             self.capture()
 
     def thermalise(self, n, adapt, gear_margin, cap_max, first=5):
-        """n evolving trajectories; the reference's dt control per chain (adapt), the PCG gear
-        chosen after `first` trajectories and raised when a solve reaches it."""
+        """n evolving trajectories; the reference's dt control per chain (adapt) and the PCG
+        gears: after `first` trajectories each solve of the trajectory gets its own cap, margin x
+        the most iterations it has taken, raised by 20% when a solve reaches it, and set once
+        more from everything seen at the end."""
         from qed_fermion_amd.trajectory import adjust_delta_t
         eng = self.eng
         queues = [collections.deque(maxlen=5) for _ in range(self.bs)]
-        seen, acc, dH = 0.0, [], []
+        seen = [0.0] * (eng.n_leap + 1)
+        acc, dH = [], []
+        gear = lambda: [min(cap_max, 10 * math.ceil(gear_margin * max(v, 1) / 10)) for v in seen]
         for k in range(n):
             eng.replay()
             th = eng.acceptance().tolist()
-            it = eng.iters.max().item()
-            seen = max(seen, it)
+            seen = [max(a, b) for a, b in zip(seen, eng.iters.max(dim=1).values.tolist())]
             acc.append(eng.accept.float().mean().item())
             dH.append((eng.H1 - eng.H0).mean().item())
             if adapt:
                 for b in range(self.bs):
                     queues[b].append(th[b])
                 adjust_delta_t(eng.dt, queues)
-            hit = (eng.status == 1).any().item()
-            if k + 1 == first or hit:
-                cap = min(cap_max, 50 * math.ceil(gear_margin * max(seen, 1) / 50))
-                if hit:
-                    cap = min(cap_max, max(cap, eng.max_iter + 100))
-                self.set_cap(cap)
+            hit = (eng.status == 1).any(dim=1).tolist()
+            if k + 1 == first:
+                self.set_caps(gear())
+            elif k + 1 > first and any(hit):
+                self.set_caps([min(cap_max, max(g, int(1.2 * c))) if h else c
+                               for g, c, h in zip(gear(), eng.max_iters, hit)])
+        self.set_caps(gear())
         return {"trajectories": n, "final_dt": eng.dt.tolist(), "acceptance_last20": sum(acc[-20:]) / max(1, len(acc[-20:])),
-                "mean_dH_last20": sum(dH[-20:]) / max(1, len(dH[-20:])), "max_iters_seen": seen, "cap": eng.max_iter}
+                "mean_dH_last20": sum(dH[-20:]) / max(1, len(dH[-20:])), "max_iters_seen_per_solve": seen,
+                "caps_per_solve": list(eng.max_iters)}
 
     def timed(self, steps, warmup):
         torch, eng = self.torch, self.eng
@@ -319,7 +328,7 @@ class Chains:
         return {"s_per_batched_trajectory": s, "s_per_chain_trajectory": s / self.bs, "steps": steps, "warmup": warmup,
                 "acceptance": acc.mean().item(), "mean_dH": d.mean().item(), "mean_exp_minus_dH": (-d).exp().mean().item(),
                 "mean_pcg_iters": its.mean().item(), "solves_at_cap": int(maxit.sum().item()),
-                "max_residual": eng.err.max().item(), "cap": eng.max_iter, "dt": eng.dt.tolist()}
+                "max_residual": eng.err.max().item(), "caps_per_solve": list(eng.max_iters), "dt": eng.dt.tolist()}
 
 
 # ------------------------------------------------------------------------------ accuracy
@@ -678,7 +687,7 @@ def main():
     log(f"[bench] thermalised {args.therm} trajectories in {time.time() - t0:.1f}s: {therm}")
     state = mixed.eng.x.clone()
     dt = mixed.eng.dt.clone()
-    cap = mixed.eng.max_iter
+    cap = list(mixed.eng.max_iters)
     head = mixed.timed(args.steps, args.warmup)
     log(f"[bench] mixed/batched: {head['s_per_chain_trajectory']:.4f} s per chain-trajectory, acceptance "
         f"{head['acceptance']:.2f}, <dH> {head['mean_dH']:.3f}, iters {head['mean_pcg_iters']:.0f}")
@@ -693,7 +702,7 @@ def main():
     del f641
     log(f"[bench] fp64 batched {head64b['s_per_chain_trajectory']:.4f}, fp64 one chain "
         f"{head641['s_per_chain_trajectory']:.4f} s per chain-trajectory")
-    acc_report = accuracy(torch, K, TrajectoryEngine, args, dev, state[:1].clone(), dt[0].item(), cap, args.rtol)
+    acc_report = accuracy(torch, K, TrajectoryEngine, args, dev, state[:1].clone(), dt[0].item(), max(cap), args.rtol)
     log(f"[bench] accuracy vs FP64: {acc_report}")
 
     # ---------------------------------------------------------- extra legs (reset fields)
@@ -725,7 +734,7 @@ def main():
     cpu = None
     if not args.skip_cpu:
         from oracle import oracle as O
-        cpu = cpu_baseline(np, O, args, state[0].cpu().numpy(), dt[0].item(), cap, rng)
+        cpu = cpu_baseline(np, O, args, state[0].cpu().numpy(), dt[0].item(), max(cap), rng)
 
     value = head["s_per_chain_trajectory"]
     t64_1 = head641["s_per_chain_trajectory"]
@@ -744,11 +753,11 @@ def main():
         "dtype": "c64+f64-reliable",
         "data": "synthetic: the reference's pi-flux + noise start, thermalised by HMC; p ~ N(0,1), R ~ randn(complex)/sqrt(2)",
         "config": {"workload": f"HMC trajectory, L={L}, Ltau={Lt}, {bs} chains batched, N_leapfrog={args.n_leap} "
-                               f"({solves} PCG solves), every solve to rtol={args.rtol} (cap {cap}), chains thermalised "
+                               f"({solves} PCG solves), every solve to rtol={args.rtol} (caps per solve {cap}), chains thermalised "
                                f"from pi flux + noise ({args.therm} trajectories, dt adapted) and evolving, precon=manual "
                                f"tau band, one hipGraph per trajectory",
                    "L": L, "Ltau": Lt, "chains": bs, "field": "pi_flux_noise_thermalised", "rtol": args.rtol,
-                   "max_iter": cap, "ru_period": args.ru_period, "parallelism": "single GPU, chains batched"},
+                   "max_iter_per_solve": cap, "ru_period": args.ru_period, "parallelism": "single GPU, chains batched"},
         "headline": head,
         "thermalisation": therm,
         "fp64_batched_s_per_trajectory": t64_b,

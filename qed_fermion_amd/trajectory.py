@@ -68,7 +68,8 @@ class TrajectoryEngine:
         self.dtau, self.compact, self.m = dtau, bool(compact), m
         self.cJ, self.cK = couplings(J, K, dtau)
         self.n_leap, self.nsub = n_leap, nsub
-        self.precision, self.max_iter, self.rtol, self.ru_period = precision, max_iter, rtol, ru_period
+        self.precision, self.rtol, self.ru_period = precision, rtol, ru_period
+        self.max_iter = max_iter
         dev = torch.device(device)
         self.device = dev
         f64, c128 = torch.float64, torch.complex128
@@ -106,6 +107,23 @@ class TrajectoryEngine:
         self.work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
         self.graph = None
 
+    # ----------------------------------------------------------- iteration caps
+    @property
+    def max_iter(self):
+        """The largest PCG iteration cap of the trajectory's solves."""
+        return max(self.max_iters)
+
+    @max_iter.setter
+    def max_iter(self, v):
+        """One cap for every solve (an int), or one per solve: solve 0 is the pseudofermion's,
+        solve k the one after leapfrog step k.  Caps are baked into a captured graph: set
+        them before capture().  The reference captures one graph per cap
+        (_MAX_ITERS_TO_CAPTURE, hmc_sampler_batch.py:305-309)."""
+        caps = [int(v)] * (self.n_leap + 1) if isinstance(v, (int, float)) else [int(c) for c in v]
+        if len(caps) != self.n_leap + 1 or min(caps) < 1:
+            raise ValueError(f"need {self.n_leap + 1} positive iteration caps, got {caps}")
+        self.max_iters = caps
+
     # ------------------------------------------------------------------ state
     def set_phi(self, phi):
         """phi: [bs, Ltau*L*L*2] (fermion layout) in any real dtype."""
@@ -116,14 +134,14 @@ class TrajectoryEngine:
         L, Lt = self.L, self.Ltau
         if self.precision == "mixed":
             kernels.pcg_mixed_solve(self.x, self.psi, L, Lt, self.dtau, ptab=self.ptab, rtol=self.rtol,
-                                    max_iter=self.max_iter, ru_period=self.ru_period, phi32=self.x32, x=self.xi,
+                                    max_iter=self.max_iters[k], ru_period=self.ru_period, phi32=self.x32, x=self.xi,
                                     work=self.work, err=self.err[k], iters=self.iters[k], status=self.status[k])
         elif self.precision == "fp64":
-            kernels.pcg_solve(self.x, self.psi, L, Lt, self.dtau, ptab=self.ptab, rtol=self.rtol, max_iter=self.max_iter,
+            kernels.pcg_solve(self.x, self.psi, L, Lt, self.dtau, ptab=self.ptab, rtol=self.rtol, max_iter=self.max_iters[k],
                               x=self.xi, work=self.work, err=self.err[k], iters=self.iters[k], status=self.status[k])
         else:
             kernels.pcg_solve(self.x32, self.psi, L, Lt, self.dtau, ptab=self.ptab, rtol=self.rtol,
-                              max_iter=self.max_iter, x=self.xi, work=self.work, err=self.err[k], iters=self.iters[k],
+                              max_iter=self.max_iters[k], x=self.xi, work=self.work, err=self.err[k], iters=self.iters[k],
                               status=self.status[k])
 
     def _force(self):

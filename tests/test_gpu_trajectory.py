@@ -134,3 +134,48 @@ def test_creutz_identity_in_equilibrium(dev):
           f"acceptance {torch.clamp(w, max=1.0).mean().item():.3f}", flush=True)
     assert abs(w.mean().item() - 1.0) < 5 * se + 0.01
     assert dH.mean().item() > -5 * (dH.std() / math.sqrt(n)).item()     # Jensen: <dH> >= 0
+
+
+def test_per_solve_caps_in_graph(dev):
+    """One iteration cap per solve of the trajectory: a replay stops each solve at its own cap
+    (QF_SOLVE_MAX_ITER where the cap is below what rtol needs) and the generous caps give the
+    eager trajectory's result."""
+    L, Lt, bs = 6, 30, 2
+    phi = make_phi(L, Lt, "gauss", seed=91, nchain=bs)
+    p = np.random.default_rng(92).standard_normal(phi.shape)
+    R = make_vec(L, Lt, seed=93, nrhs=bs)
+    eng = _engine(L, Lt, bs, "mixed", max_iter=400, rtol=1e-10, n_leap=2, delta_t=0.03)
+    with pytest.raises(ValueError):
+        eng.max_iter = [400, 400]           # needs n_leap + 1 caps
+
+    def load():
+        eng.set_phi(_to_fermion_layout(phi))
+        eng.p.copy_(torch.from_numpy(p.reshape(bs, -1)))
+        eng.R.copy_(torch.from_numpy(R))
+
+    load()
+    eng.propose(draw=False)
+    torch.cuda.synchronize()
+    need = eng.iters.max(dim=1).values.tolist()
+    assert (eng.status == 0).all() and max(need) < 400
+    ref = eng.x.clone()
+    caps = [int(need[0]) + 5, 7, int(need[2]) + 20]   # solve 2 sees a field moved by a capped force
+    eng.max_iter = caps
+    assert eng.max_iter == max(caps)
+    load()
+    eng.capture(n_warmup=1, draw=False, accept=False)
+    for rep in range(2):
+        load()
+        eng.replay()
+        torch.cuda.synchronize()
+        assert eng.status[0].tolist() == [0, 0] and eng.status[1].tolist() == [1, 1], rep
+        assert eng.iters[1].tolist() == [7.0, 7.0]
+        assert (eng.iters <= torch.tensor(caps, dtype=torch.float64, device=dev).view(-1, 1)).all()
+    eng.max_iter = [int(v) + 5 for v in need]
+    load()
+    eng.capture(n_warmup=1, draw=False, accept=False)
+    load()
+    eng.replay()
+    torch.cuda.synchronize()
+    assert (eng.status == 0).all()
+    assert relerr(eng.x.cpu().numpy(), ref.cpu().numpy()) < 1e-13
