@@ -313,7 +313,9 @@ __global__ void cg_finalize_kernel(const double* rr_part, int n, const CgState* 
 // Adaptive reliable update of the mixed-precision solver (checked every few
 // iterations, entirely on the device so the solve stays graph-capturable):
 // a chain updates when its iterated residual has dropped by delta since the
-// last true residual, or ru_max iterations have passed, or at the end (force).
+// last true residual, or has reached rtol (a chain may only stop on a true
+// residual, so this check ends the solve without iterating on to the next
+// delta drop), or ru_max iterations have passed, or at the end (force).
 // The update flushes x_hi += x_lo, x_lo = 0 here; the FP64 residual kernel and
 // the refresh kernel that follow read the per-chain flag and exit if it is 0.
 // State is parity double-buffered per check (prev -> out) because every block
@@ -336,6 +338,8 @@ struct RuArgs {
     int iter;                  // iterations completed
     int ru_max;
     double delta2;
+    const double* normb;       // |b| per chain
+    double rtol2;              // rtol^2 (0: no convergence trigger)
     int force;
 };
 
@@ -352,7 +356,9 @@ __global__ void __launch_bounds__(256) ru_decide_flush_kernel(RuArgs A) {
     const double rr_last = prev_updated ? s1[0] : A.rr_prev[rho];
     const double since = (double)A.iter - A.it_prev[rho];
     const bool active = A.st[rho].act != 0.0;
-    const bool doit = A.force || (active && (since >= A.ru_max || rr_now < A.delta2 * rr_last));
+    const double nb = A.normb[rho];
+    const bool at_rtol = rr_now <= A.rtol2 * nb * nb;
+    const bool doit = A.force || (active && (since >= A.ru_max || rr_now < A.delta2 * rr_last || at_rtol));
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         A.flag_out[rho] = doit ? 1 : 0;
         A.rr_out[rho] = rr_last;
