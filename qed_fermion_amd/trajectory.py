@@ -199,9 +199,19 @@ class TrajectoryEngine:
         self.metropolis()
 
     # ---------------------------------------------------------------- graph
+    # A captured graph holds every PCG iteration up to its cap (two kernels each, plus the
+    # reliable-update checks); ending the capture of a graph of about 330,000 kernel nodes
+    # crashed the HIP runtime on the host, 120,000 captured fine.
+    MAX_GRAPH_ITERATIONS = 65536
+
     def capture(self, n_warmup=1, pool=None, draw=True, accept=True):
         """Capture one trajectory (draws, solves, forces, leapfrog, accept) as a hipGraph.
-        draw=False leaves p and R to the caller; accept=False stops at the proposal."""
+        draw=False leaves p and R to the caller; accept=False stops at the proposal.  The
+        solves' caps together may hold at most MAX_GRAPH_ITERATIONS iterations: choose caps per
+        solve (max_iter as a list) from the iterations the solves need."""
+        if sum(self.max_iters) > self.MAX_GRAPH_ITERATIONS:
+            raise ValueError(f"a trajectory graph of {sum(self.max_iters)} PCG iterations (caps {self.max_iters}) exceeds "
+                             f"{self.MAX_GRAPH_ITERATIONS}; lower the caps per solve")
         def body():
             self.propose(draw=draw)
             if accept:

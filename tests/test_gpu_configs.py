@@ -71,3 +71,44 @@ def test_c2_graph_trajectory(dev):
     assert relerr(eng.x[0].cpu().numpy(), x1) < 1e-8
     assert relerr(eng.p[0].cpu().numpy(), p1) < 1e-8
     assert abs((eng.H1[0] - eng.H0[0]).item() - (H1 - H0)) < 1e-6
+
+
+def test_c2_graph_trajectory_at_1e12(dev):
+    """C2 at the configuration's own bar: the 40-step FP64 trajectory replayed from its hipGraph,
+    every solve converged to 1e-14, ends within 1e-12 relative of the oracle's in field and
+    momenta (every 64th entry and 16 Gaussian projections, tests/golden/make_c2_trajectory.py),
+    with H0 and H1 within 1e-12 relative.  At 1e-14 the solver's rounding moves the oracle's own
+    final state by less than the bar."""
+    import os
+    from qed_fermion_amd.trajectory import TrajectoryEngine
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c2_trajectory_rtol1e-14.npz"))
+    L, Lt, nl, stride, nproj, pseed = (int(v) for v in d["meta"])
+    print("[C2 1e-12]", flush=True)
+    phi = make_phi(L, Lt, "gauss", seed=2401)
+    p = np.random.default_rng(2402).standard_normal(phi.shape)
+    R = make_vec(L, Lt, seed=2403)
+    eng = TrajectoryEngine(L, Lt, 1, precision="fp64", max_iter=4000, rtol=float(d["rtol"]), n_leap=nl,
+                           delta_t=float(d["dt"]), device="cuda")
+    _load(eng, phi, p, R)
+    eng.propose(draw=False)                 # eager: the iterations each solve needs
+    torch.cuda.synchronize()
+    need = eng.iters[:, 0].tolist()
+    print(f"[C2 1e-12] iterations per solve {min(need):.0f}-{max(need):.0f}", flush=True)
+    assert (eng.status == 0).all()
+    eng.max_iter = [int(v) + 20 for v in need]   # caps per solve, as the trajectory's gears
+    _load(eng, phi, p, R)
+    eng.capture(n_warmup=1, draw=False, accept=False)
+    _load(eng, phi, p, R)
+    eng.replay()
+    torch.cuda.synchronize()
+    assert (eng.status == 0).all() and eng.err.max().item() <= 1e-14
+    x = eng.x[0].cpu().numpy().ravel()
+    pm = eng.p[0].cpu().numpy().ravel()
+    assert relerr(x[::stride], d["x_sub"]) < 1e-12
+    assert relerr(pm[::stride], d["p_sub"]) < 1e-12
+    G = np.random.default_rng(pseed).standard_normal((nproj, x.size))
+    assert np.abs(G @ x - d["x_proj"]).max() < 1e-12 * float(d["x_norm"])
+    assert np.abs(G @ pm - d["p_proj"]).max() < 1e-12 * float(d["p_norm"])
+    H0, H1 = float(d["H0"]), float(d["H1"])
+    assert abs(eng.H0[0].item() - H0) < 1e-12 * abs(H0)
+    assert abs(eng.H1[0].item() - H1) < 1e-12 * abs(H1)
