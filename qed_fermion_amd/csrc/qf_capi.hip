@@ -451,7 +451,7 @@ int pcg_iterate(const R* phi, long long phi_cs, const cplx<R>* b_in, cplx<R>* x,
         a.rr_prev = w.ru_rr + pc * nrhs; a.rr_out = w.ru_rr + c * nrhs;
         a.it_prev = w.ru_it + pc * nrhs; a.it_out = w.ru_it + c * nrhs;
         a.flag_prev = w.ru_flag + pc * nrhs; a.flag_out = w.ru_flag + c * nrhs;
-        a.st = w.st + cur * nrhs; a.iter = iters_done; a.ru_max = ru_period; a.delta2 = 1e-2; a.force = force;
+        a.st = w.st + cur * nrhs; a.iter = iters_done; a.ru_max = ru_period; a.delta2 = 1e-4; a.force = force;
         a.normb = (const double*)w.normb; a.rtol2 = rtol * rtol;
         hipLaunchKernelGGL(ru_decide_flush_kernel, dim3(ru_blocks, nrhs), dim3(256), 0, s, a);
         QF_LAUNCHED();
