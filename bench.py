@@ -83,7 +83,7 @@ def parse():
                     help="chains (own fields) of the roofline M^dag M; 32 at L=36 is a 358 MB working set")
     ap.add_argument("--roof-reps", type=int, default=20)
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--cpu-c2", action="store_true", help="also time the C2 trajectory (L=24, 40 leapfrog) on the CPU")
+    ap.add_argument("--cpu-c2", action="store_true", help="time the full C2 trajectory (L=24, 40 leapfrog) on the CPU instead of its first 4 steps")
     ap.add_argument("--no-tune", action="store_true", help="use the built-in tilings instead of the autotuner")
     ap.add_argument("--probe", choices=["per_chain", "per_chain_hop", "shared", "bond_bond", "pcg"], default=None,
                     help="per_chain (from phi) / per_chain_hop / shared: a calibration copy, then the roofline M^dag M "
@@ -472,6 +472,35 @@ def extras(torch, K, args, dev):
                                           "max_residual": eng.err.max().item(), "tiling": tune66}
     log(f"[bench] L=66 20-step mixed trajectory: {t66:.3f} s")
     del eng
+    # -- the plain-method configurations: C1 (L=6, Ltau=60, uniform field, 20 leapfrog steps, Metropolis) and
+    #    C2 (L=24, Ltau=240, sigma=0.3 field, 40 leapfrog steps), one FP64 chain, every solve to 1e-10, one
+    #    hipGraph per trajectory with caps per solve from an eager trajectory on the same field
+    for name, (Lc, Ltc, kind, nl, dtc) in (("c1_L6_fp64_20step", (6, 60, "uniform", 20, 0.06)),
+                                            ("c2_L24_fp64_40step", (24, 240, "gauss", 40, 0.02))):
+        Nc = Lc * Lc * Ltc
+        phc = rng.uniform(-np.pi, np.pi, 2 * Nc) if kind == "uniform" else 0.3 * rng.standard_normal(2 * Nc)
+        phc = torch.from_numpy(phc).view(1, -1).to(dev)
+        eng = TrajectoryEngine(Lc, Ltc, 1, precision="fp64", dtau=0.1, J=1.0, K=1.0, compact=False, m=1.0, delta_t=dtc,
+                               n_leap=nl, max_iter=4000, rtol=1e-10, device=dev)
+        eng.set_phi(phc)
+        eng.propose()
+        torch.cuda.synchronize()
+        need = eng.iters[:, 0].tolist()
+        eng.max_iter = [min(4000, int(1.2 * v) + 20) for v in need]
+        graph = sum(eng.max_iters) <= eng.MAX_GRAPH_ITERATIONS   # else eager launches
+        eng.set_phi(phc)
+        if graph:
+            eng.capture(n_warmup=1)
+
+        def one():
+            eng.set_phi(phc)
+            eng.replay() if graph else eng.trajectory()
+        tc = timed(torch, one, reps=3)
+        out[name] = {"s_per_trajectory": tc, "solves": nl + 1, "dt": dtc, "field": kind, "graph": graph,
+                     "iterations_per_solve_eager": [min(need), max(need)], "caps_total": sum(eng.max_iters),
+                     "max_residual": eng.err.max().item(), "solves_at_cap": int((eng.status == 1).sum().item())}
+        log(f"[bench] {name}: {tc * 1e3:.1f} ms per trajectory")
+        del eng
     return out
 
 
@@ -490,7 +519,8 @@ def cpu_model():
 def cpu_baseline(np, O, args, phi_h, dt, cap, rng):
     """The FP64 OpenMP oracle on this host: one full headline trajectory (6 solves to rtol with
     the same cap and early exit, forces, the boson leapfrog) on a thermalised chain, C1
-    trajectories (and C2 with --cpu-c2), and M^dag M rates (48 bytes per site in complex128)."""
+    trajectories, C2's first 4 leapfrog steps (its full trajectory with --cpu-c2), and M^dag M
+    rates (48 bytes per site in complex128)."""
     L, Lt = args.L, args.Ltau
     N = L * L * Lt
     tab = O.precon_table(Lt)
@@ -516,15 +546,22 @@ def cpu_baseline(np, O, args, phi_h, dt, cap, rng):
         O.trajectory(ph1, rng.standard_normal(ph1.shape), R1, L1, Lt1, dt=0.06, n_leap=20, table=O.precon_table(Lt1),
                      rtol=1e-10, max_iter=4000)
     out["c1_s_per_trajectory"] = (time.perf_counter() - t0) / reps
+    # C2: L=24, Ltau=240, sigma=0.3 field, dt 0.02: the full 40-step trajectory with --cpu-c2, otherwise a
+    # bounded sample of its first 4 leapfrog steps (5 of its 41 solves)
+    L2, Lt2 = 24, 240
+    N2 = L2 * L2 * Lt2
+    ph2 = rng.normal(0.0, 0.3, (Lt2, L2, L2, 2))
+    R2 = (rng.standard_normal(N2) + 1j * rng.standard_normal(N2)) / 2.0
+    nl2 = 40 if args.cpu_c2 else 4
+    t0 = time.perf_counter()
+    O.trajectory(ph2, rng.standard_normal(ph2.shape), R2, L2, Lt2, dt=0.02, n_leap=nl2, table=O.precon_table(Lt2),
+                 rtol=1e-10, max_iter=4000)
+    t2 = time.perf_counter() - t0
     if args.cpu_c2:
-        L2, Lt2 = 24, 240
-        N2 = L2 * L2 * Lt2
-        ph2 = rng.normal(0.0, 0.3, (Lt2, L2, L2, 2))
-        R2 = (rng.standard_normal(N2) + 1j * rng.standard_normal(N2)) / 2.0
-        t0 = time.perf_counter()
-        O.trajectory(ph2, rng.standard_normal(ph2.shape), R2, L2, Lt2, dt=0.06, n_leap=40, table=O.precon_table(Lt2),
-                     rtol=1e-10, max_iter=4000)
-        out["c2_s_per_trajectory"] = time.perf_counter() - t0
+        out["c2_s_per_trajectory"] = t2
+    else:
+        out["c2_sample"] = {"leapfrog_steps": nl2, "solves": nl2 + 1, "s": t2,
+                            "s_per_trajectory_scaled_by_solves": t2 * 41 / (nl2 + 1)}
     rates = {}
     for (Lc, Ltc) in ((6, 60), (24, 240), (36, 360)):
         Nc = Lc * Lc * Ltc
@@ -539,7 +576,7 @@ def cpu_baseline(np, O, args, phi_h, dt, cap, rng):
         rates[f"L{Lc}_Ltau{Ltc}"] = {"ms": s * 1e3, "GBps": 48 * Nc / s / 1e9}
     out["mdagm_c128"] = rates
     log(f"[bench] cpu baseline: {t_head:.1f} s per headline trajectory on {out['cores']} threads ({out['cpu_model']}), "
-        f"C1 {out['c1_s_per_trajectory']:.3f} s")
+        f"C1 {out['c1_s_per_trajectory']:.3f} s, C2 {nl2} steps {t2:.1f} s")
     return out
 
 

@@ -368,3 +368,64 @@ def test_wrappers_reject_wrong_buffers(dev):
         K.boson_force(phi, L, Lt, 1.0, 1.0, Fp=torch.zeros(2 * N, dtype=torch.float32, device=dev))
     with pytest.raises(K.QfError):
         K.hamiltonian(phi, L, Lt, 1.0, 1.0, H=torch.zeros(3, dtype=torch.float64, device=dev))
+
+
+def test_force_gathered_over_the_c_shim(dev):
+    """The reference's own force assembly over the _C shim (Level-1 drop-in): force_f_fast
+    (hmc_sampler_batch.py:2170-2227) gathers df_dot_bs1 (1987-2006) products of
+    _C.b_vec_per_tau partial products (interm=True) per slice.  Restated here over this build's
+    shim and checkerboard lists, it must give the fused adjoint force kernel's result for the
+    same xi: this pins the shim's order and conjugation convention of the partial products,
+    which no reference output shows.  The assembly pairs partial k with partial 4 - k, so it cannot
+    tell the order 0..4 from its reverse; two partials swapped fail it."""
+    from qed_fermion_amd import _C
+    from qed_fermion_amd import kernels as K
+    L, Lt, dtau, t = 4, 10, 0.1, 1.0
+    hmc = _sampler(L, Lt, 1, "fp64")
+    Vs = L * L
+    phi = torch.from_numpy(make_phi(L, Lt, "gauss", seed=95).reshape(1, -1)).to(dev)
+    xi = torch.from_numpy(make_vec(L, Lt, seed=96)).to(dev)
+    bos = phi.view(Lt, 2 * Vs)
+    xt = xi.view(Lt, Vs)
+
+    def df_dot(blist, lft, rgt, il, jl, group1=False):
+        v = (math.sinh(t * dtau) if group1 else math.sinh(t * dtau / 2)) * torch.exp(1j * (blist + math.pi / 2))
+        return lft[il] * rgt[jl] * v + lft[jl] * rgt[il] * v.conj()
+
+    def assemble(partials_order):
+        F = torch.empty(Lt, 2 * Vs, dtype=torch.float64, device=dev)
+        groups = [(hmc.boson_idx_list_1, hmc.i_list_1, hmc.j_list_1), (hmc.boson_idx_list_2, hmc.i_list_2, hmc.j_list_2),
+                  (hmc.boson_idx_list_3, hmc.i_list_3, hmc.j_list_3), (hmc.boson_idx_list_4, hmc.i_list_4, hmc.j_list_4)]
+        for tau in range(Lt):
+            b = bos[tau]
+            xc, xn = xt[tau], xt[(tau + 1) % Lt]
+            part = lambda v: _C.b_vec_per_tau(b, v, L, dtau, True)[partials_order]
+            Bx = _C.b_vec_per_tau(b, xc, L, dtau, False)
+            nl = part(xn.conj())          # xi_n_lft_conj
+            cr = part(xc)                 # xi_c_rgt
+            bc = part(Bx.conj())          # B_xi_conj
+            s = -1.0 if tau < Lt - 1 else 1.0
+            (i1, il1, jl1), (i2, il2, jl2), (i3, il3, jl3), (i4, il4, jl4) = groups
+            F[tau, i1] = 2 * torch.real(df_dot(b[i1], nl[2].conj(), cr[2], il1, jl1, True) * s
+                                        + df_dot(b[i1], bc[2].conj(), cr[2], il1, jl1, True))
+            F[tau, i2] = 2 * torch.real((df_dot(b[i2], nl[1].conj(), cr[3], il2, jl2) + df_dot(b[i2], nl[3].conj(), cr[1], il2, jl2)) * s
+                                        + df_dot(b[i2], bc[1].conj(), cr[3], il2, jl2) + df_dot(b[i2], bc[3].conj(), cr[1], il2, jl2))
+            F[tau, i3] = 2 * torch.real((df_dot(b[i3], nl[0].conj(), cr[4], il3, jl3) + df_dot(b[i3], nl[4].conj(), cr[0], il3, jl3)) * s
+                                        + df_dot(b[i3], bc[0].conj(), cr[4], il3, jl3) + df_dot(b[i3], bc[4].conj(), cr[0], il3, jl3))
+            F[tau, i4] = 2 * torch.real((df_dot(b[i4], xn.conj(), cr[5], il4, jl4) + df_dot(b[i4], nl[5].conj(), xc, il4, jl4)) * s
+                                        + df_dot(b[i4], Bx.conj(), cr[5], il4, jl4) + df_dot(b[i4], bc[5].conj(), xc, il4, jl4))
+        return F.view(1, -1)
+
+    Fk = K.fermion_force(phi, xi, L, Lt, dtau, t)
+    Fg = assemble(slice(None))
+    scale = Fk.abs().max().item()
+    assert scale > 0
+    # the reference returns +2 Re(...), which is -dS_f/dphi up to the sign of its derivative convention
+    err = min((Fg - Fk).abs().max().item(), (Fg + Fk).abs().max().item()) / scale
+    assert err < 1e-10, err
+    sign = 1.0 if (Fg - Fk).abs().max().item() < (Fg + Fk).abs().max().item() else -1.0
+    print(f"[shim force] max rel diff {err:.2e}, sign {sign:+.0f}", flush=True)
+    same = assemble(torch.tensor([4, 3, 2, 1, 0, 5], device=dev))    # the reverse order: indistinguishable
+    assert (same - sign * Fk).abs().max().item() / scale < 1e-10
+    wrong = assemble(torch.tensor([1, 0, 2, 3, 4, 5], device=dev))   # two partials swapped
+    assert (wrong - sign * Fk).abs().max().item() / scale > 1e-3
