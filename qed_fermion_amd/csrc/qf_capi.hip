@@ -219,12 +219,16 @@ int launch_mdm_k(const MdmArgs<R>& a0, int nrhs, hipStream_t s) {
 
 // CG step A: complex64 computes its coefficients from phi (hardware sincos:
 // measured about 5% faster per iteration than reading the 16-byte-per-site hop
-// table), complex128 reads the solve's hop table.  The FP64 residual of the
-// reliable updates reads phi; plain applications take either.
+// table), except with 2 x 6 patches, whose from-phi form spills 66 VGPRs (22
+// from the table: the L=66 trajectory 5.6% faster); complex128 reads the solve's
+// hop table.  The FP64 residual of the reliable updates reads phi; plain
+// applications take either.
+template <typename R> constexpr bool cg_uses_hop(int by) { return sizeof(R) == 8 || by == 6; }
+
 template <typename R, int BY, int MODE>
 int launch_mdm_by(const MdmArgs<R>& a, int nrhs, hipStream_t s) {
     if constexpr (MODE == MDM_CG) {
-        if constexpr (sizeof(R) == 4) return launch_mdm_k<R, BY, MODE, false>(a, nrhs, s);
+        if constexpr (sizeof(R) == 4 && BY != 6) return launch_mdm_k<R, BY, MODE, false>(a, nrhs, s);
         if (!a.hop) return fail(QF_ERR_INVALID, "CG step without a hop table");
         return launch_mdm_k<R, BY, MODE, true>(a, nrhs, s);
     } else if constexpr (MODE == MDM_RESID) {
@@ -419,9 +423,10 @@ int pcg_iterate(const R* phi, long long phi_cs, const cplx<R>* b_in, cplx<R>* x,
     m.phi = phi; m.phi_cs = phi_cs; m.vs = N; m.L = L; m.Lt = Lt; m.T = t.T; m.hp = hp;
     m.nB = t.nB; m.normb = w.normb; m.rtol = rtol; m.dot_part = w.pap; m.z = (const cplx<R>*)w.Z; m.y = (cplx<R>*)w.OP;
     const long long ps = (long long)nrhs * t.nB;   // partial slot stride
-    // complex128: hop table of the field(s), read by the iterations instead of
-    // recomputing sincos; complex64 iterations take phi directly
-    if constexpr (sizeof(R) == 8) {
+    // complex128 (and complex64 with 2 x 6 patches): hop table of the field(s),
+    // read by the iterations instead of recomputing sincos; other complex64
+    // iterations take phi directly
+    if (cg_uses_hop<R>(t.by)) {
         const int nchain = phi_cs ? nrhs : 1;
         int rc0 = launch_hop_table<R>(phi, phi_cs, (cplx<R>*)w.HOP, nchain, L, Lt, hp.dtau_, hp.t_, s);
         if (rc0) return rc0;
