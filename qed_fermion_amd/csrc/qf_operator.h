@@ -306,7 +306,8 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
                         make_float4((float)Z[2 * r].x, (float)Z[2 * r].y, (float)Z[2 * r + 1].x, (float)Z[2 * r + 1].y);
             } else {
                 store_patch_g<R, BY>(Z, yout + (long long)tau * Vs, po, L);
-                dsum += redot_patch<R, BY>(Y, Y);
+                // Re(p^dag A p) = |M p|^2 partials of PCG step A; a plain M^dag M has no use for them
+                if constexpr (MODE == MDM_CG) dsum += redot_patch<R, BY>(Y, Y);
             }
         }
     };
