@@ -450,9 +450,15 @@ def extras(torch, K, args, dev):
     ptr_ = plan.precon_table(dtype=torch.float32, device=dev)
     t_rg = timed(torch, lambda: K.pcg_solve_ragged(plan, phr, br, ptab=ptr_, rtol=0.0, max_iter=it))
     t_rop = timed(torch, lambda: K.apply_ragged(plan, phr, br), reps=10)
+    phr64 = phr.double()
+    br64 = br.to(torch.complex128)
+    t_rgm = timed(torch, lambda: K.pcg_mixed_solve_ragged(plan, phr64, br64, ptab=ptr_, rtol=0.0, max_iter=it,
+                                                          ru_period=args.ru_period, phi32=phr))
     out["ragged_8_lattices"] = {"L": Ls, "solve_s": t_rg, "iterations": it, "ms_per_iteration": t_rg / it * 1e3,
-                                "mdagm_ms": t_rop * 1e3, "mdagm_GBps": 24 * plan.nsites / t_rop / 1e9}
-    log(f"[bench] ragged batch: {t_rg:.3f} s per {it}-iteration solve, one M^dag M launch {t_rop * 1e3:.3f} ms")
+                                "mdagm_ms": t_rop * 1e3, "mdagm_GBps": 24 * plan.nsites / t_rop / 1e9,
+                                "mixed_solve_s": t_rgm, "mixed_ms_per_iteration": t_rgm / it * 1e3}
+    log(f"[bench] ragged batch: {t_rg:.3f} s per {it}-iteration solve ({t_rgm:.3f} s mixed), one M^dag M launch "
+        f"{t_rop * 1e3:.3f} ms")
     # -- differentiation at full size: L=66, Ltau=660, a 20-step mixed trajectory
     L, Lt = 66, 660
     N = L * L * Lt

@@ -176,6 +176,16 @@ int qf_pcg_solve_ragged(int prec, const void* plan_host, const void* plan_dev, c
                         double t, const void* ptab, double rtol, int max_iter, void* work, size_t work_bytes, double* err_out,
                         double* iters_out, int* status_out, void* stream);
 
+/* The mixed-precision solver of qf_pcg_mixed_solve over a ragged batch: complex64
+ * iterations with FP64 dot products and adaptive FP64 reliable updates (at most every
+ * ru_period iterations), phi64 / phi32 the concatenated fields, b and x complex128, ptab32
+ * the concatenated float band tables.  A lattice stops only on an FP64 true residual, and
+ * err_out is that residual.  No counterpart in the reference, whose batches share one L. */
+size_t qf_pcg_mixed_ragged_workspace_bytes(const void* plan_host);
+int qf_pcg_mixed_solve_ragged(const void* plan_host, const void* plan_dev, const double* phi64, const float* phi32, const void* b, void* x,
+                              double dtau, double t, const float* ptab32, double rtol, int max_iter, int ru_period, void* work,
+                              size_t work_bytes, double* err_out, double* iters_out, int* status_out, void* stream);
+
 /* Fermion force Ft = -dS_f/dphi given xi = (M^dag M)^{-1} psi: the second half of
  * force_f_fast (hmc_sampler_batch.py:2174-2227, per-tau _C.b_vec_per_tau calls and
  * df_dot gathers 1969-2006) fused into one launch.  Ft has the phi layout. */

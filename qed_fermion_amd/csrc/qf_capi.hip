@@ -1030,13 +1030,180 @@ int qf_apply_ragged(int op, int prec, const void* plan_host, const void* plan_de
     return fail(QF_ERR_INVALID, "prec %d", prec);
 }
 
+}  // extern "C"
+
+namespace {
+// Workspace of a ragged PCG solve (mixed: complex64 iterations with FP64 reliable updates).
+struct RgWork {
+    void* R[2];
+    void* P[2];
+    void* Z;
+    void* OP;
+    void* HOP;       // complex128 only
+    void* XLO;       // mixed only: the complex64 iterate
+    void* BLO;       // mixed only: b in complex64
+    double* rzp;
+    double* rrp;
+    double* pap;
+    double* rp64;    // mixed only: FP64 residual partials (operator layout)
+    CgState* st;
+    double* normb;
+    int* brk;
+    double* ru_rr;   // mixed only: [2][nprob]
+    double* ru_it;
+    int* ru_flag;
+    size_t bytes;
+};
+
+RgWork carve_ragged(void* base, int prec, bool mixed, const RaggedHead* h) {
+    RgWork w{};
+    const size_t vb = al256((size_t)h->nsites * (prec == QF_PREC_F64 ? 16 : 8));
+    const int np = h->nprob;
+    char* p = static_cast<char*>(base);
+    size_t off = 0;
+    auto take = [&](size_t n) { void* r = p ? p + off : nullptr; off += al256(n); return r; };
+    w.R[0] = take(vb); w.R[1] = take(vb); w.P[0] = take(vb); w.P[1] = take(vb); w.Z = take(vb); w.OP = take(vb);
+    w.HOP = prec == QF_PREC_F64 ? take(2 * vb) : nullptr;
+    w.XLO = mixed ? take(vb) : nullptr;
+    w.BLO = mixed ? take(vb) : nullptr;
+    w.rzp = (double*)take(sizeof(double) * 2 * h->nB_tot);
+    w.rrp = (double*)take(sizeof(double) * 2 * h->nB_tot);
+    w.pap = (double*)take(sizeof(double) * h->nA_tot);
+    w.rp64 = mixed ? (double*)take(sizeof(double) * h->nA_tot) : nullptr;
+    w.st = (CgState*)take(sizeof(CgState) * 2 * np);
+    w.normb = (double*)take(sizeof(double) * np);
+    w.brk = (int*)take(sizeof(int) * np);
+    w.ru_rr = mixed ? (double*)take(sizeof(double) * 2 * np) : nullptr;
+    w.ru_it = mixed ? (double*)take(sizeof(double) * 2 * np) : nullptr;
+    w.ru_flag = mixed ? (int*)take(sizeof(int) * 2 * np) : nullptr;
+    w.bytes = off;
+    return w;
+}
+
+// PCG over a ragged batch: one operator launch and one update launch per iteration for
+// every lattice (the recurrence of pcg_iterate, one chain per lattice, x updated every
+// iteration).  ru_period > 0 (complex64): the mixed-precision solver of pcg_iterate, with
+// x the complex64 iterate, reliable updates checked every 10 iterations into x_hi (FP64
+// residual from phi64 and b_hi), and chains stopping only on a true residual.
+template <typename R>
+int pcg_ragged(const RaggedHead* h, const Prob* pd, const R* phi, const cplx<R>* b, cplx<R>* x, double dtau, double t, const R* ptab,
+               double rtol, int max_iter, const RgWork& w, double* err_out, double* iters_out, int* status_out, hipStream_t s,
+               int ru_period = 0, const double* phi64 = nullptr, const double2* b_hi = nullptr, double2* x_hi = nullptr) {
+    const Hop<R> hp = make_hop<R>(dtau, t);
+    const int np = h->nprob;
+    const long long ps = h->nB_tot;
+    const bool mixed = ru_period > 0;
+    if constexpr (sizeof(R) == 8) {
+        hipLaunchKernelGGL((hop_table_ragged_kernel<R>), dim3((unsigned)std::min<long long>((h->nsites + 255) / 256, 16384)),
+                           dim3(256), 0, s, phi, (cplx<R>*)w.HOP, pd, np, h->nsites, std::tanh(t * dtau), std::tanh(0.5 * t * dtau));
+        QF_LAUNCHED();
+    }
+    CguArgs<R> u{};
+    u.ptab = ptab; u.prob = pd; u.nprob = np;
+    u.rold = b; u.x = x; u.rnew = (cplx<R>*)w.R[1]; u.z = (cplx<R>*)w.Z;
+    u.st_init = w.st + np; u.rz_part = w.rzp + ps; u.rr_part = w.rrp + ps; u.brk = w.brk;
+    int rc = launch_cgu_ragged<R, CGU_INIT>(u, *h, s);
+    if (rc) return rc;
+    MdmArgs<R> m{};
+    m.phi = phi; m.hop = (const cplx<R>*)w.HOP; m.hp = hp; m.prob = pd; m.nprob = np; m.nrhs = 1;
+    m.normb = w.normb; m.rtol = rtol; m.dot_part = w.pap; m.z = (const cplx<R>*)w.Z; m.y = (cplx<R>*)w.OP; m.may_stop = 1;
+    m.brk = w.brk;
+
+    // mixed precision: the reliable-update check after iteration i uses parity check & 1
+    constexpr int kRuCheck = 10;
+    const Prob* ph = reinterpret_cast<const Prob*>(h + 1);   // host copy of the plan
+    long long nmax = 1;
+    for (int k = 0; k < np; ++k) nmax = std::max(nmax, (long long)ph[k].L * ph[k].L * ph[k].Lt);
+    const int ru_blocks = (int)std::max(1LL, std::min((nmax + 1023) / 1024, 512LL));
+    const Hop<double> hp64 = make_hop<double>(dtau, t);
+    int check = 0;
+    auto reliable_update = [&](int iters_done, int slot, int force, int cur) -> int {
+        RuArgs a{};
+        const int c = check & 1, pc = c ^ 1;
+        a.xhi = x_hi; a.xlo = (float2*)x; a.prob = pd;
+        a.rr_part = w.rrp + slot * ps; a.rp64 = w.rp64;
+        a.rr_prev = w.ru_rr + pc * np; a.rr_out = w.ru_rr + c * np;
+        a.it_prev = w.ru_it + pc * np; a.it_out = w.ru_it + c * np;
+        a.flag_prev = w.ru_flag + pc * np; a.flag_out = w.ru_flag + c * np;
+        a.st = w.st + cur * np; a.iter = iters_done; a.ru_max = ru_period; a.delta2 = 1e-4; a.force = force;
+        a.normb = (const double*)w.normb; a.rtol2 = rtol * rtol;
+        hipLaunchKernelGGL(ru_decide_flush_kernel, dim3(ru_blocks, np), dim3(256), 0, s, a);
+        QF_LAUNCHED();
+        MdmArgs<double> r{};
+        r.phi = phi64; r.x = x_hi; r.hp = hp64; r.prob = pd; r.nprob = np; r.nrhs = 1;
+        r.b = b_hi; r.r_lo = (float2*)w.R[cur]; r.dot_part = w.rp64; r.ru_flag = w.ru_flag + c * np;
+        int rc2 = launch_mdm_ragged<double, MDM_RESID, false>(r, *h, s);
+        if (rc2) return rc2;
+        ++check;
+        return QF_OK;
+    };
+
+    for (int i = 0; i < max_iter; ++i) {
+        const int cur = i & 1, prv = cur ^ 1;
+        m.iter = i;
+        m.may_stop = 1;
+        m.stop_flag = nullptr;
+        if (mixed && i > 0) {
+            if (i % kRuCheck == 0) m.stop_flag = w.ru_flag + ((check - 1) & 1) * np;
+            else m.may_stop = 0;
+        }
+        m.rz_part = w.rzp + prv * ps; m.rr_part = w.rrp + prv * ps;
+        m.st_prev = w.st + prv * np; m.st_out = w.st + cur * np;
+        m.pold = (const cplx<R>*)w.P[prv]; m.pnew = (cplx<R>*)w.P[cur];
+        rc = sizeof(R) == 8 ? launch_mdm_ragged<R, MDM_CG, true>(m, *h, s) : launch_mdm_ragged<R, MDM_CG, false>(m, *h, s);
+        if (rc) return rc;
+        if (mixed && i == 0) {
+            hipLaunchKernelGGL(ru_init_kernel, dim3((np + 63) / 64), dim3(64), 0, s, (const double*)w.normb, w.ru_rr + np, w.ru_it + np,
+                               w.ru_flag + np, np);
+            QF_LAUNCHED();
+        }
+        u.rold = (const cplx<R>*)w.R[prv]; u.op = (const cplx<R>*)w.OP; u.p = (const cplx<R>*)w.P[cur];
+        u.x = x; u.rnew = (cplx<R>*)w.R[cur]; u.z = (cplx<R>*)w.Z;
+        u.pap = w.pap; u.st = w.st + cur * np;
+        u.rz_part = w.rzp + cur * ps; u.rr_part = w.rrp + cur * ps;
+        u.ru_flag = nullptr;
+        rc = launch_cgu_ragged<R, CGU_ITER>(u, *h, s);
+        if (rc) return rc;
+        if (mixed && (i + 1) % kRuCheck == 0 && i + 1 < max_iter) {
+            // reliable update where needed: x_hi += x_lo; r = b - A64 x_hi (FP64); z = P r
+            rc = reliable_update(i + 1, cur, 0, cur);
+            if (rc) return rc;
+            CguArgs<R> f = u;
+            f.rold = (const cplx<R>*)w.R[cur];
+            f.ru_flag = w.ru_flag + ((check - 1) & 1) * np;
+            rc = launch_cgu_ragged<R, CGU_REFRESH>(f, *h, s);
+            if (rc) return rc;
+        }
+    }
+    const int last = (max_iter - 1) & 1;
+    if (mixed) {
+        rc = reliable_update(max_iter, last, 1, last);
+        if (rc) return rc;
+        hipLaunchKernelGGL(cg_finalize_kernel, dim3(np), dim3(64), 0, s, (const double*)w.rp64, 0, (const CgState*)(w.st + last * np),
+                           (const double*)w.normb, rtol, 1, err_out, iters_out, status_out, pd);
+    } else {
+        hipLaunchKernelGGL(cg_finalize_kernel, dim3(np), dim3(64), 0, s, (const double*)(w.rrp + last * ps), 0,
+                           (const CgState*)(w.st + last * np), (const double*)w.normb, rtol, 0, err_out, iters_out, status_out, pd);
+    }
+    QF_LAUNCHED();
+    return QF_OK;
+}
+}  // namespace
+
+extern "C" {
+
 size_t qf_pcg_ragged_workspace_bytes(int prec, const void* plan_host) {
     if (!plan_host || (prec != QF_PREC_F32 && prec != QF_PREC_F64)) return 0;
     const RaggedHead* h = static_cast<const RaggedHead*>(plan_host);
     if (h->nprob < 1) return 0;
-    const size_t vb = al256((size_t)h->nsites * (prec == QF_PREC_F64 ? 16 : 8));
-    return 6 * vb + 2 * vb + al256(sizeof(double) * 2 * h->nB_tot) * 2 + al256(sizeof(double) * h->nA_tot) +
-           al256(sizeof(CgState) * 2 * h->nprob) + al256(sizeof(double) * h->nprob) + al256(sizeof(int) * h->nprob);
+    return carve_ragged(nullptr, prec, false, h).bytes;
+}
+
+size_t qf_pcg_mixed_ragged_workspace_bytes(const void* plan_host) {
+    if (!plan_host) return 0;
+    const RaggedHead* h = static_cast<const RaggedHead*>(plan_host);
+    if (h->nprob < 1) return 0;
+    return carve_ragged(nullptr, QF_PREC_F32, true, h).bytes;
 }
 
 int qf_pcg_solve_ragged(int prec, const void* plan_host, const void* plan_dev, const void* phi, const void* b, void* x, double dtau,
@@ -1048,69 +1215,38 @@ int qf_pcg_solve_ragged(int prec, const void* plan_host, const void* plan_dev, c
     if (rc) return rc;
     if (!phi || !b || !x || !work || !err_out || !iters_out) return fail(QF_ERR_INVALID, "null pointer");
     if (max_iter < 1) return fail(QF_ERR_INVALID, "max_iter=%d must be >= 1", max_iter);
-    const size_t need = qf_pcg_ragged_workspace_bytes(prec, plan_host);
-    if (need == 0 || work_bytes < need) return fail(QF_ERR_INVALID, "workspace of %zu bytes, need %zu", work_bytes, need);
+    if (prec != QF_PREC_F32 && prec != QF_PREC_F64) return fail(QF_ERR_INVALID, "prec %d", prec);
+    const RgWork w = carve_ragged(work, prec, false, h);
+    if (work_bytes < w.bytes) return fail(QF_ERR_INVALID, "workspace of %zu bytes, need %zu", work_bytes, w.bytes);
     hipStream_t s = (hipStream_t)stream;
-    auto run = [&](auto tag) -> int {
-        using R = decltype(tag);
-        const size_t vb = al256((size_t)h->nsites * sizeof(cplx<R>));
-        char* p = static_cast<char*>(work);
-        size_t off = 0;
-        auto take = [&](size_t n) { void* r = p + off; off += al256(n); return r; };
-        void* Rb[2] = {take(vb), take(vb)};
-        void* Pb[2] = {take(vb), take(vb)};
-        void* Z = take(vb);
-        void* OP = take(vb);
-        void* HOP = take(2 * vb);
-        double* rzp = (double*)take(sizeof(double) * 2 * h->nB_tot);
-        double* rrp = (double*)take(sizeof(double) * 2 * h->nB_tot);
-        double* pap = (double*)take(sizeof(double) * h->nA_tot);
-        CgState* st = (CgState*)take(sizeof(CgState) * 2 * h->nprob);
-        double* normb = (double*)take(sizeof(double) * h->nprob);
-        int* brk = (int*)take(sizeof(int) * h->nprob);
-        const Hop<R> hp = make_hop<R>(dtau, t);
-        const int np = h->nprob;
-        const long long ps = h->nB_tot;
-        if constexpr (sizeof(R) == 8) {
-            hipLaunchKernelGGL((hop_table_ragged_kernel<R>), dim3((unsigned)std::min<long long>((h->nsites + 255) / 256, 16384)),
-                               dim3(256), 0, s, (const R*)phi, (cplx<R>*)HOP, pd, np, h->nsites, std::tanh(t * dtau),
-                               std::tanh(0.5 * t * dtau));
-            QF_LAUNCHED();
-        }
-        CguArgs<R> u{};
-        u.ptab = (const R*)ptab; u.prob = pd; u.nprob = np;
-        u.rold = (const cplx<R>*)b; u.x = (cplx<R>*)x; u.rnew = (cplx<R>*)Rb[1]; u.z = (cplx<R>*)Z;
-        u.st_init = st + np; u.rz_part = rzp + ps; u.rr_part = rrp + ps; u.brk = brk;
-        int rc2 = launch_cgu_ragged<R, CGU_INIT>(u, *h, s);
-        if (rc2) return rc2;
-        MdmArgs<R> m{};
-        m.phi = (const R*)phi; m.hop = (const cplx<R>*)HOP; m.hp = hp; m.prob = pd; m.nprob = np; m.nrhs = 1;
-        m.normb = normb; m.rtol = rtol; m.dot_part = pap; m.z = (const cplx<R>*)Z; m.y = (cplx<R>*)OP; m.may_stop = 1;
-        m.brk = brk;
-        for (int i = 0; i < max_iter; ++i) {
-            const int cur = i & 1, prv = cur ^ 1;
-            m.iter = i;
-            m.rz_part = rzp + prv * ps; m.rr_part = rrp + prv * ps;
-            m.st_prev = st + prv * np; m.st_out = st + cur * np;
-            m.pold = (const cplx<R>*)Pb[prv]; m.pnew = (cplx<R>*)Pb[cur];
-            rc2 = sizeof(R) == 8 ? launch_mdm_ragged<R, MDM_CG, true>(m, *h, s) : launch_mdm_ragged<R, MDM_CG, false>(m, *h, s);
-            if (rc2) return rc2;
-            u.rold = (const cplx<R>*)Rb[prv]; u.op = (const cplx<R>*)OP; u.p = (const cplx<R>*)Pb[cur];
-            u.x = (cplx<R>*)x; u.rnew = (cplx<R>*)Rb[cur]; u.z = (cplx<R>*)Z;
-            u.pap = pap; u.st = st + cur * np;
-            u.rz_part = rzp + cur * ps; u.rr_part = rrp + cur * ps;
-            rc2 = launch_cgu_ragged<R, CGU_ITER>(u, *h, s);
-            if (rc2) return rc2;
-        }
-        const int last = (max_iter - 1) & 1;
-        hipLaunchKernelGGL(cg_finalize_kernel, dim3(np), dim3(64), 0, s, (const double*)(rrp + last * ps), 0,
-                           (const CgState*)(st + last * np), (const double*)normb, rtol, 0, err_out, iters_out, status_out, pd);
-        QF_LAUNCHED();
-        return QF_OK;
-    };
-    if (prec == QF_PREC_F32) return run(float{});
-    if (prec == QF_PREC_F64) return run(double{});
-    return fail(QF_ERR_INVALID, "prec %d", prec);
+    if (prec == QF_PREC_F32)
+        return pcg_ragged<float>(h, pd, (const float*)phi, (const float2*)b, (float2*)x, dtau, t, (const float*)ptab, rtol, max_iter, w,
+                                 err_out, iters_out, status_out, s);
+    return pcg_ragged<double>(h, pd, (const double*)phi, (const double2*)b, (double2*)x, dtau, t, (const double*)ptab, rtol, max_iter, w,
+                              err_out, iters_out, status_out, s);
+}
+
+int qf_pcg_mixed_solve_ragged(const void* plan_host, const void* plan_dev, const double* phi64, const float* phi32, const void* b, void* x,
+                              double dtau, double t, const float* ptab32, double rtol, int max_iter, int ru_period, void* work,
+                              size_t work_bytes, double* err_out, double* iters_out, int* status_out, void* stream) {
+    const RaggedHead* h;
+    const Prob* pd;
+    int rc = ragged_head(plan_host, plan_dev, &h, &pd);
+    if (rc) return rc;
+    if (!phi64 || !phi32 || !b || !x || !work || !err_out || !iters_out) return fail(QF_ERR_INVALID, "null pointer");
+    if (max_iter < 1) return fail(QF_ERR_INVALID, "max_iter=%d must be >= 1", max_iter);
+    if (ru_period < 1) return fail(QF_ERR_INVALID, "reliable-update period %d must be >= 1", ru_period);
+    const RgWork w = carve_ragged(work, QF_PREC_F32, true, h);
+    if (work_bytes < w.bytes) return fail(QF_ERR_INVALID, "workspace of %zu bytes, need %zu", work_bytes, w.bytes);
+    hipStream_t s = (hipStream_t)stream;
+    const long long n = h->nsites;
+    const dim3 g1(std::min<long long>((n + 255) / 256, 8192));
+    hipLaunchKernelGGL(demote_kernel, g1, dim3(256), 0, s, (const double2*)b, (float2*)w.BLO, n);
+    QF_LAUNCHED();
+    hipLaunchKernelGGL(zero_kernel, g1, dim3(256), 0, s, (double*)x, 2 * n);
+    QF_LAUNCHED();
+    return pcg_ragged<float>(h, pd, phi32, (const float2*)w.BLO, (float2*)w.XLO, dtau, t, ptab32, rtol, max_iter, w, err_out, iters_out,
+                             status_out, s, ru_period, phi64, (const double2*)b, (double2*)x);
 }
 
 int qf_fermion_force(int prec, const void* phi, long long phi_cs, const void* xi, void* Ft, int nrhs, int L, int Ltau, double dtau,

@@ -281,9 +281,9 @@ __global__ void cg_finalize_kernel(const double* rr_part, int n, const CgState* 
     const int rho = blockIdx.x;
     double sums[1];
     long long off = (long long)rho * n;
-    if (prob != nullptr) {   // ragged batch: this lattice's update partials
-        off = prob[rho].pb0;
-        n = prob[rho].nB;
+    if (prob != nullptr) {   // ragged batch: this lattice's update (or, true_res, operator) partials
+        off = true_res ? prob[rho].pa0 : prob[rho].pb0;
+        n = true_res ? prob[rho].nA : prob[rho].nB;
     }
     const double* src[1] = {rr_part + off};
     sum_partials<1>(src, n, sums, red);
@@ -341,17 +341,19 @@ struct RuArgs {
     const double* normb;       // |b| per chain
     double rtol2;              // rtol^2 (0: no convergence trigger)
     int force;
+    const Prob* prob;          // ragged batch: per-lattice vector offsets and partial-sum ranges (nullptr: uniform)
 };
 
 __global__ void __launch_bounds__(256) ru_decide_flush_kernel(RuArgs A) {
     __shared__ double red[64];
     const int rho = blockIdx.y;
     const bool prev_updated = A.flag_prev[rho] != 0;
+    const Prob* P = A.prob != nullptr ? A.prob + rho : nullptr;
     double s0[1], s1[1];
-    const double* src0[1] = {A.rr_part + (long long)rho * A.nB};
-    const double* src1[1] = {A.rp64 + (long long)rho * A.nA64};
-    sum_partials<1>(src0, A.nB, s0, red);
-    sum_partials<1>(src1, prev_updated ? A.nA64 : 0, s1, red);   // true residual of the last update
+    const double* src0[1] = {A.rr_part + (P ? (long long)P->pb0 : (long long)rho * A.nB)};
+    const double* src1[1] = {A.rp64 + (P ? (long long)P->pa0 : (long long)rho * A.nA64)};
+    sum_partials<1>(src0, P ? P->nB : A.nB, s0, red);
+    sum_partials<1>(src1, prev_updated ? (P ? P->nA : A.nA64) : 0, s1, red);   // true residual of the last update
     const double rr_now = s0[0];
     const double rr_last = prev_updated ? s1[0] : A.rr_prev[rho];
     const double since = (double)A.iter - A.it_prev[rho];
@@ -365,9 +367,10 @@ __global__ void __launch_bounds__(256) ru_decide_flush_kernel(RuArgs A) {
         A.it_out[rho] = doit ? (double)A.iter : A.it_prev[rho];
     }
     if (!doit) return;
-    double2* xh = A.xhi + rho * A.N;
-    float2* xl = A.xlo + rho * A.N;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < A.N; i += (long long)gridDim.x * blockDim.x) {
+    const long long n = P ? (long long)P->L * P->L * P->Lt : A.N;
+    double2* xh = A.xhi + (P ? P->voff : rho * A.N);
+    float2* xl = A.xlo + (P ? P->voff : rho * A.N);
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         const float2 l = xl[i];
         double2 h = xh[i];
         h.x += (double)l.x;

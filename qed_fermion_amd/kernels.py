@@ -445,3 +445,33 @@ def pcg_solve_ragged(plan, phi, b, dtau=0.1, t=1.0, ptab=None, rtol=1e-9, max_it
     call("qf_pcg_solve_ragged", prec, plan.host.ctypes.data, ptr(plan.dev), ptr(phi), ptr(b), ptr(x), float(dtau), float(t),
          ptr(tab), float(rtol), int(max_iter), ptr(work), work.numel(), ptr(err), ptr(iters), ptr(st), stream_handle())
     return x, iters, err
+
+
+def pcg_mixed_solve_ragged(plan, phi64, b, dtau=0.1, t=1.0, ptab=None, rtol=1e-10, max_iter=400, ru_period=50, phi32=None,
+                           x=None, status=None):
+    """Mixed-precision PCG over a ragged batch (complex64 iterations, FP64 reliable updates at most
+    every ``ru_period`` iterations).  phi64 float64 [2 * sum N_k], b complex128 [sum N_k].  Returns
+    (x complex128, iters, err_true) with one iters / err entry per lattice; ``status`` (int32
+    [lattices], optional) gets the QF_SOLVE_* codes."""
+    _check_dev(phi64, b, ptab)
+    if b.numel() != plan.nsites or phi64.numel() != 2 * plan.nsites:
+        raise QfError(_lib.QF_ERR_INVALID, "ragged vector / field sizes do not match the plan")
+    if b.dtype != torch.complex128 or phi64.dtype != torch.float64:
+        raise QfError(_lib.QF_ERR_INVALID, "the mixed ragged solve takes complex128 b and float64 phi")
+    if ptab is not None and ptab.numel() != 13 * sum(plan.Ltaus):
+        raise QfError(_lib.QF_ERR_INVALID, "preconditioner tables do not match the plan")
+    dev = b.device
+    b = _buf(b, "b", torch.complex128, plan.nsites, dev)
+    phi32 = phi64.to(torch.float32) if phi32 is None else _buf(phi32, "phi32", torch.float32, phi64.numel(), dev)
+    nbytes = _lib.lib().qf_pcg_mixed_ragged_workspace_bytes(plan.host.ctypes.data)
+    work = _workspace(("ragged_mixed", tuple(plan.Ls), tuple(plan.Ltaus), dev), nbytes, dev)
+    x = torch.empty_like(b) if x is None else _buf(x, "x", torch.complex128, b.numel(), dev)
+    n = len(plan.Ls)
+    err = torch.empty(n, dtype=torch.float64, device=dev)
+    iters = torch.empty(n, dtype=torch.float64, device=dev)
+    tab = ptab.to(torch.float32).contiguous() if ptab is not None else None
+    st = _status_buf(status, n, dev)
+    call("qf_pcg_mixed_solve_ragged", plan.host.ctypes.data, ptr(plan.dev), ptr(phi64), ptr(phi32), ptr(b), ptr(x), float(dtau),
+         float(t), ptr(tab), float(rtol), int(max_iter), int(ru_period), ptr(work), work.numel(), ptr(err), ptr(iters), ptr(st),
+         stream_handle())
+    return x, iters, err

@@ -106,6 +106,12 @@ def test_ragged_plan_layout():
     assert head[2] == 256                          # 15 x 15 = 225 threads -> 4 waves
     nsites = np.frombuffer(plan[32:48].tobytes(), dtype=np.int64)
     assert nsites[0] == int(np.sum(Ls.astype(np.int64) ** 2 * Lts)) and nsites[1] == 13 * int(np.sum(Lts))
+    # workspaces: the mixed solve adds x_lo, b_lo and the FP64 partials; complex128 adds the hop table
+    nb32 = lib.qf_pcg_ragged_workspace_bytes(0, plan.ctypes.data)
+    nbm = lib.qf_pcg_mixed_ragged_workspace_bytes(plan.ctypes.data)
+    nb64 = lib.qf_pcg_ragged_workspace_bytes(1, plan.ctypes.data)
+    assert 0 < nb32 < nbm and nb32 < nb64
+    assert nbm - nb32 >= 2 * 8 * int(nsites[0])
     odd = np.array([12, 13], dtype=np.int32)
     assert lib.qf_ragged_plan(2, odd.ctypes.data, Lts.ctypes.data, plan.ctypes.data, nb) == _lib.QF_ERR_ODD_L
     assert lib.qf_ragged_plan(n, Ls.ctypes.data, Lts.ctypes.data, plan.ctypes.data, 8) == _lib.QF_ERR_INVALID

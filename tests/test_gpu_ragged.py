@@ -62,3 +62,37 @@ def test_pcg_ragged_matches_per_lattice_solves(dev):
         assert err[k].item() <= 1e-10 and iters[k].item() < 2000, (k, err[k].item())
         r = O.apply_MdagM(pk.cpu().numpy().reshape(Lt, L, L, 2), xs[k].cpu().numpy(), L, Lt)
         assert relerr(r, bk.view(-1).cpu().numpy()) <= 1.01e-10, k
+
+
+def test_mixed_pcg_ragged(dev):
+    """The mixed-precision solver over the ragged set: every lattice reaches the oracle-recomputed
+    FP64 true residual (err reports it), its solution agrees with the FP64 ragged solve within the
+    mixed-vs-FP64 bar of 1e-7, and its iteration count is within a reliable-update check period of
+    the uniform mixed solver's on that lattice alone."""
+    from qed_fermion_amd import kernels as K
+    print("[ragged mixed pcg]", flush=True)
+    plan = K.RaggedPlan(LS, [10 * L for L in LS])
+    phis, vecs = _batch("gauss", 21)
+    phi = torch.from_numpy(np.concatenate(phis)).to(dev, torch.float64)
+    R = torch.from_numpy(np.concatenate(vecs)).to(dev, torch.complex128)
+    b = torch.cat([K.apply("Mdag", p.view(1, -1), r.view(1, -1), L, 10 * L).view(-1)
+                   for p, r, L in zip(plan.split(phi.view(-1, 2)), plan.split(R), LS)]).contiguous()
+    status = torch.full((len(LS),), -1, dtype=torch.int32, device=dev)
+    x, iters, err = K.pcg_mixed_solve_ragged(plan, phi, b, ptab=plan.precon_table(dtype=torch.float32, device=dev), rtol=1e-10,
+                                             max_iter=2000, ru_period=50, status=status)
+    x64, it64, _ = K.pcg_solve_ragged(plan, phi, b, ptab=plan.precon_table(device=dev), rtol=1e-12, max_iter=3000)
+    torch.cuda.synchronize()
+    assert status.tolist() == [0] * len(LS)
+    xs, x64s = plan.split(x), plan.split(x64)
+    for k, L in enumerate(LS):
+        Lt = 10 * L
+        pk = plan.split(phi.view(-1, 2))[k].reshape(1, -1)
+        bk = plan.split(b)[k].view(1, -1)
+        r = O.apply_MdagM(pk.cpu().numpy().reshape(Lt, L, L, 2), xs[k].cpu().numpy(), L, Lt)
+        true_res = relerr(r, bk.view(-1).cpu().numpy())
+        assert true_res <= 1.01e-10, (k, true_res)
+        assert abs(err[k].item() - true_res) < 1e-3 * true_res + 1e-15, (k, err[k].item(), true_res)
+        assert relerr(xs[k].cpu().numpy(), x64s[k].cpu().numpy()) < 1e-7, k
+        _, itk, _ = K.pcg_mixed_solve(pk, bk, L, Lt, ptab=K.precon_table(Lt, dtype=torch.float32), rtol=1e-10, max_iter=2000,
+                                      ru_period=50)
+        assert abs(iters[k].item() - itk.item()) <= 10, (k, iters[k].item(), itk.item())
