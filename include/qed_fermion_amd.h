@@ -52,7 +52,8 @@ enum {
     QF_SOLVE_CONVERGED = 0, /* relative residual <= rtol */
     QF_SOLVE_MAX_ITER = 1,  /* max_iter iterations without reaching rtol (the reference's fixed budget) */
     QF_SOLVE_NONFINITE = 2, /* NaN or Inf in the residual, Re(r^dag z) or Re(p^dag A p): a NaN in phi or b */
-    QF_SOLVE_BREAKDOWN = 3  /* Re(p^dag A p) <= 0: the operator is not positive definite on p */
+    QF_SOLVE_BREAKDOWN = 3, /* Re(p^dag A p) <= 0: the operator is not positive definite on p */
+    QF_SOLVE_ABORTED = 4    /* qf_pcg_solve_persistent only: a grid barrier timed out (err_out is NaN) */
 };
 
 enum { QF_OP_M = 1, QF_OP_MDAG = 2, QF_OP_B = 3, QF_OP_MDAGM = 4, QF_OP_BPART0 = 16 };
@@ -144,6 +145,31 @@ size_t qf_pcg_noalloc_max_bytes(void);
 int qf_pcg_solve_noalloc(int prec, const void* phi, long long phi_cs, const void* b, void* x, int nrhs, int L, int Ltau, double dtau,
                          double t, const void* ptab, double rtol, int max_iter, double* err_out, double* iters_out, int* status_out,
                          void* stream);
+
+/* qf_pcg_solve as ONE persistent cooperative launch: the same recurrence and
+ * the same per-iteration work (the operator and update kernels' bodies), with
+ * the work-groups taking each phase's items in turn and a grid barrier
+ * between the phases, so the whole solve is one dispatch for any number of
+ * right-hand sides (the multi-RHS solve of stochastic_estimator.py:427-433).
+ * The launch leaves as soon as no chain is active (the reference's fixed
+ * max_iter is the upper bound; frozen chains do nothing, so err_out,
+ * iters_out, status_out and x equal those of qf_pcg_solve up to the order of
+ * the partial sums).  ptab must not be NULL.  Every barrier wait is bounded
+ * (10 s): a timed-out solve marks every chain QF_SOLVE_ABORTED.  work: at
+ * least qf_pcg_persistent_workspace_bytes(prec, nrhs, L, Ltau) bytes, whose
+ * last 256 bytes (barrier counters) must be zero before the first call on
+ * that workspace (every call leaves them zero).  Calls sharing a workspace
+ * must not overlap.  L up to 64 (work-groups of at most 512 threads); returns
+ * QF_ERR_UNSUPPORTED otherwise.  Not for stream capture. */
+size_t qf_pcg_persistent_workspace_bytes(int prec, int nrhs, int L, int Ltau);
+/* Time limit of each grid-barrier wait of qf_pcg_solve_persistent, in microseconds
+ * (default 10 s); returns the previous value.  A limit of 0 makes every wait time out at
+ * once, which exercises the abort path (every work-group leaves, every chain is marked
+ * QF_SOLVE_ABORTED) without a fault.  No counterpart in the reference. */
+long long qf_set_persistent_timeout_us(long long us);
+int qf_pcg_solve_persistent(int prec, const void* phi, long long phi_cs, const void* b, void* x, int nrhs, int L, int Ltau, double dtau,
+                            double t, const void* ptab, double rtol, int max_iter, void* work, size_t work_bytes, double* err_out,
+                            double* iters_out, int* status_out, void* stream);
 
 /* Mixed-precision PCG (same recurrence and outputs): complex64 vectors and
  * operator with FP64 dot products, and every ru_period iterations a reliable

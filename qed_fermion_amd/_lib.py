@@ -30,8 +30,9 @@ QF_SOLVE_CONVERGED = 0
 QF_SOLVE_MAX_ITER = 1
 QF_SOLVE_NONFINITE = 2
 QF_SOLVE_BREAKDOWN = 3
+QF_SOLVE_ABORTED = 4
 SOLVE_STATUS_NAMES = {0: "converged", 1: "max_iter reached", 2: "non-finite residual or step (NaN/Inf in phi or b)",
-                      3: "breakdown (Re(p^dag A p) <= 0)"}
+                      3: "breakdown (Re(p^dag A p) <= 0)", 4: "aborted (a grid barrier of the persistent solve timed out)"}
 
 QF_OBS_T1 = 0
 QF_OBS_T21 = 1
@@ -76,6 +77,9 @@ SIGNATURES = {
     "qf_pcg_solve": (_I, [_I, _P, _LL, _P, _P, _I, _I, _I, _D, _D, _P, _D, _I, _P, _SZ, _P, _P, _P, _P]),
     "qf_pcg_noalloc_max_bytes": (_SZ, []),
     "qf_pcg_solve_noalloc": (_I, [_I, _P, _LL, _P, _P, _I, _I, _I, _D, _D, _P, _D, _I, _P, _P, _P, _P]),
+    "qf_pcg_persistent_workspace_bytes": (_SZ, [_I, _I, _I, _I]),
+    "qf_set_persistent_timeout_us": (_LL, [_LL]),
+    "qf_pcg_solve_persistent": (_I, [_I, _P, _LL, _P, _P, _I, _I, _I, _D, _D, _P, _D, _I, _P, _SZ, _P, _P, _P, _P]),
     "qf_pcg_mixed_solve": (_I, [_P, _P, _LL, _P, _P, _I, _I, _I, _D, _D, _P, _D, _I, _I, _P, _SZ, _P, _P, _P, _P]),
     "qf_ragged_plan_bytes": (_SZ, [_I]),
     "qf_ragged_plan": (_I, [_I, _P, _P, _P, _SZ]),

@@ -25,6 +25,7 @@
 #include "qf_force.h"
 #include "qf_obs.h"
 #include "qf_operator.h"
+#include "qf_persist.h"
 
 using namespace qf;
 
@@ -68,6 +69,7 @@ struct ShapeTuning {
 std::vector<ShapeTuning> g_shape_tune;
 std::mutex g_tune_mu;
 std::atomic<int> g_paired_x{1};   // qf_set_paired_x_updates
+std::atomic<long long> g_pers_timeout_us{10000000};   // qf_set_persistent_timeout_us
 
 // Work-groups of up to 512 threads get 256 VGPRs per lane; only the 2x2
 // patch may use 1024-thread work-groups (L up to 64).
@@ -535,6 +537,85 @@ int pcg_iterate(const R* phi, long long phi_cs, const cplx<R>* b_in, cplx<R>* x,
     return QF_OK;
 }
 
+// ------------------------------------------------------------ persistent solve
+// Tiling of the persistent launch: the operator phase sets the work-group
+// (blk_of for the patch height of the solve's tiling, at most 512 threads);
+// the update phase uses kPersTX x (blk * kPersSPT / kPersTX) tiles.
+struct PersPlan {
+    Tiling t;
+    int blk, ntile;
+};
+
+PersPlan pers_plan(int prec, int nrhs, int L, int Lt) {
+    PersPlan p;
+    p.t = tiling(prec, nrhs, L, Lt);
+    p.blk = (int)blk_of(L, p.t.by).x;
+    p.t.TB = p.blk * kPersSPT / kPersTX;
+    p.t.nsx = (L * L + kPersTX - 1) / kPersTX;
+    p.ntile = (Lt + p.t.TB - 1) / p.t.TB;
+    p.t.nB = p.t.nsx * p.ntile;
+    return p;
+}
+
+// Barrier counters after the solver workspace (zero before the first use; every solve
+// leaves them zero).
+constexpr size_t kPersBarBytes = 256;
+
+template <typename R, int BY>
+int launch_pers(PcgPersArgs<R>& a, int blk, hipStream_t s) {
+    constexpr bool PIPE = mdm_pipe<R, BY>();
+    constexpr bool HOPT = cg_uses_hop<R>(BY);
+    auto k = pcg_persistent_kernel<R, BY, HOPT, PIPE>;
+    const size_t lds = std::max<size_t>(lds_bytes_c<R>(BY, blk), cgu_stage_bytes<R>(blk * kPersSPT / kPersTX, kPersTX));
+    int rc = set_lds(k, lds);
+    if (rc) return rc;
+    int dev = 0, ncu = 0, per = 0, khz = 0;
+    QF_HIP(hipGetDevice(&dev));
+    QF_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    QF_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    QF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, blk, lds));
+    if (per < 1) return fail(QF_ERR_UNSUPPORTED, "persistent solve: no work-group of %d threads fits a CU", blk);
+    const long long items = std::max((long long)a.nrhs * a.nA, (long long)a.nrhs * a.nsx * a.ntile);
+    const int G = (int)std::max(1LL, std::min<long long>((long long)per * ncu, items));
+    a.timeout_ticks = g_pers_timeout_us.load() * (long long)khz / 1000LL;   // wall-clock ticks per wait
+    void* args[] = {&a};
+    QF_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k), dim3(G), dim3(blk), args, (unsigned)lds, s));
+    return QF_OK;
+}
+
+template <typename R>
+int pcg_persistent(const R* phi, long long phi_cs, const cplx<R>* b, cplx<R>* x, int nrhs, int L, int Lt, const Hop<R>& hp, const R* ptab,
+                   double rtol, int max_iter, const CgWork& w, void* bar, const PersPlan& pl, hipStream_t s, double* err_out,
+                   double* iters_out, int* status_out) {
+    const Tiling& t = pl.t;
+    const long long N = (long long)L * L * Lt;
+    PcgPersArgs<R> a{};
+    a.u.vs = N; a.u.Vs = L * L; a.u.Lt = Lt; a.u.TB = t.TB; a.u.ptab = ptab; a.u.nA = t.nA;
+    a.m.phi = phi; a.m.phi_cs = phi_cs; a.m.vs = N; a.m.L = L; a.m.Lt = Lt; a.m.T = t.T; a.m.hp = hp; a.m.nrhs = nrhs;
+    a.m.nB = t.nB; a.m.normb = w.normb; a.m.rtol = rtol; a.m.dot_part = w.pap; a.m.z = (const cplx<R>*)w.Z; a.m.y = (cplx<R>*)w.OP;
+    a.nrhs = nrhs; a.nA = t.nA; a.nsx = t.nsx; a.ntile = pl.ntile; a.max_iter = max_iter; a.paired = g_paired_x.load() != 0;
+    a.b = b; a.x = x;
+    a.R2[0] = (cplx<R>*)w.R[0]; a.R2[1] = (cplx<R>*)w.R[1]; a.P2[0] = (cplx<R>*)w.P[0]; a.P2[1] = (cplx<R>*)w.P[1];
+    a.Z = (cplx<R>*)w.Z; a.OP = (cplx<R>*)w.OP;
+    a.rzp = w.rzp; a.rrp = w.rrp; a.pap = w.pap; a.ps = (long long)nrhs * t.nB;
+    a.st = w.st; a.normb = w.normb; a.xal = w.xal; a.brk = w.brk;
+    if (cg_uses_hop<R>(t.by)) {
+        a.phi = phi; a.phi_cs = phi_cs; a.hop = (cplx<R>*)w.HOP; a.nchain = phi_cs ? nrhs : 1;
+        a.th1 = std::tanh(hp.t_ * hp.dtau_); a.th2 = std::tanh(0.5 * hp.t_ * hp.dtau_);
+        a.m.hop = (const cplx<R>*)w.HOP;
+        a.m.hop_cs = phi_cs ? 2 * N : 0;
+    }
+    a.rtol = rtol; a.err_out = err_out; a.iters_out = iters_out; a.status_out = status_out;
+    unsigned* c = static_cast<unsigned*>(bar);
+    a.gb.count = c; a.gb.done = c + 16; a.gb.abort = c + 32;
+    switch (t.by) {
+        case 2: return launch_pers<R, 2>(a, pl.blk, s);
+        case 4: return launch_pers<R, 4>(a, pl.blk, s);
+        case 6: return launch_pers<R, 6>(a, pl.blk, s);
+    }
+    return fail(QF_ERR_UNSUPPORTED, "patch height %d", t.by);
+}
+
 // ------------------------------------------------------------ preconditioner
 // Float32 roundings of the manual band constants of
 // preconditioners_orig/precon_manual.py:37-167 (the reference stores them as
@@ -701,6 +782,8 @@ int qf_clear_tuning(void) {
 }
 
 int qf_set_paired_x_updates(int on) { return g_paired_x.exchange(on ? 1 : 0); }
+
+long long qf_set_persistent_timeout_us(long long us) { return g_pers_timeout_us.exchange(us < 0 ? 0 : us); }
 
 int qf_get_tiling(int prec, int nrhs, int L, int Ltau, int* out4) {
     int rc = check_lattice(L, Ltau, nrhs);
@@ -929,6 +1012,45 @@ int qf_pcg_solve_noalloc(int prec, const void* phi, long long phi_cs, const void
     // contents need no clearing
     return qf_pcg_solve(prec, phi, phi_cs, b, x, nrhs, L, Ltau, dtau, t, ptab, rtol, max_iter, work, kQfScratchBytes, err_out,
                         iters_out, status_out, stream);
+}
+
+size_t qf_pcg_persistent_workspace_bytes(int prec, int nrhs, int L, int Ltau) {
+    if (check_lattice(L, Ltau, nrhs)) return 0;
+    if (prec != QF_PREC_F32 && prec != QF_PREC_F64) {
+        fail(QF_ERR_INVALID, "prec %d", prec);
+        return 0;
+    }
+    const PersPlan pl = pers_plan(prec, nrhs, L, Ltau);
+    if (pl.blk > 512) {
+        fail(QF_ERR_UNSUPPORTED, "L=%d: the persistent solve takes work-groups of at most 512 threads", L);
+        return 0;
+    }
+    const Tiling t64 = tiling(QF_PREC_F64, nrhs, L, Ltau);
+    return carve(nullptr, prec, 0, nrhs, L, Ltau, pl.t, t64).bytes + kPersBarBytes;
+}
+
+int qf_pcg_solve_persistent(int prec, const void* phi, long long phi_cs, const void* b, void* x, int nrhs, int L, int Ltau, double dtau,
+                            double t, const void* ptab, double rtol, int max_iter, void* work, size_t work_bytes, double* err_out,
+                            double* iters_out, int* status_out, void* stream) {
+    int rc = check_lattice(L, Ltau, nrhs);
+    if (rc) return rc;
+    if (!phi || !b || !x || !work || !err_out || !iters_out) return fail(QF_ERR_INVALID, "null pointer");
+    if (!ptab) return fail(QF_ERR_INVALID, "the persistent solve needs the preconditioner table");
+    if (max_iter < 1 || max_iter > 1000000) return fail(QF_ERR_INVALID, "max_iter=%d out of range [1, 1000000]", max_iter);
+    if (prec != QF_PREC_F32 && prec != QF_PREC_F64) return fail(QF_ERR_INVALID, "prec %d", prec);
+    const PersPlan pl = pers_plan(prec, nrhs, L, Ltau);
+    if (pl.blk > 512) return fail(QF_ERR_UNSUPPORTED, "L=%d: the persistent solve takes work-groups of at most 512 threads", L);
+    const Tiling t64 = tiling(QF_PREC_F64, nrhs, L, Ltau);
+    const CgWork w = carve(work, prec, 0, nrhs, L, Ltau, pl.t, t64);
+    if (work_bytes < w.bytes + kPersBarBytes)
+        return fail(QF_ERR_INVALID, "workspace of %zu bytes, need %zu", work_bytes, w.bytes + kPersBarBytes);
+    void* bar = static_cast<char*>(work) + w.bytes;
+    hipStream_t s = (hipStream_t)stream;
+    if (prec == QF_PREC_F32)
+        return pcg_persistent<float>((const float*)phi, phi_cs, (const float2*)b, (float2*)x, nrhs, L, Ltau, make_hop<float>(dtau, t),
+                                     (const float*)ptab, rtol, max_iter, w, bar, pl, s, err_out, iters_out, status_out);
+    return pcg_persistent<double>((const double*)phi, phi_cs, (const double2*)b, (double2*)x, nrhs, L, Ltau, make_hop<double>(dtau, t),
+                                  (const double*)ptab, rtol, max_iter, w, bar, pl, s, err_out, iters_out, status_out);
 }
 
 int qf_pcg_mixed_solve(const double* phi64, const float* phi32, long long phi_cs, const void* b, void* x, int nrhs, int L, int Ltau,

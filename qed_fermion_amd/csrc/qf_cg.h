@@ -66,41 +66,63 @@ struct CguArgs {
 // residual of the tile's TT + 12 slices (the 13-point tau window) is staged in
 // LDS, so no thread marches in tau and all loads of a block are in flight at
 // once.  Slices outside [0, Lt) of the window wrap cyclically (P is cyclic).
-// Partial sums: one (rz, rr) pair per block, blk = blockIdx.x + nsx * blockIdx.y.
+// Partial sums: one (rz, rr) pair per block, blk = bx + nsx * by.
+// TT = 0: the tile height follows the work-group, TT = blockDim.x * SPT / TX
+// (at most 512 threads), for a persistent launch whose work-group size is set
+// by the operator phase; the staged residual and band rows then live in the
+// caller's LDS at stage (cgu_stage_bytes).  The body takes the tile
+// coordinates (bx, by) of rhs bz in a grid of gx x gy tiles per rhs (lin:
+// linear tile index of a ragged batch); consecutive calls in one work-group
+// must be separated by a work-group barrier.
+template <typename R>
+__host__ __device__ constexpr int cgu_cst() { return (13 * (int)sizeof(R) + 15) / 16 * 16 / (int)sizeof(R); }
+template <typename R>
+__host__ __device__ constexpr unsigned cgu_ring_bytes(int tt, int tx) { return (unsigned)((tt + 12) * tx) * (unsigned)sizeof(cplx<R>); }
+template <typename R>
+__host__ __device__ constexpr unsigned cgu_stage_bytes(int tt, int tx) {
+    return (cgu_ring_bytes<R>(tt, tx) + 15u) / 16u * 16u + (unsigned)(tt * cgu_cst<R>()) * (unsigned)sizeof(R);
+}
+
 template <typename R, int MODE, bool PRE, int TX, int TT, int SPT>
-__global__ void __launch_bounds__(TX * TT / SPT) cg_update_kernel(CguArgs<R> A) {
+__device__ __forceinline__ void cgu_body(const CguArgs<R>& A, int lin, int bx_, int by_, int bz_, int gx, int gy,
+                                         unsigned char* stage = nullptr) {
     constexpr int H = PRE ? 12 : 0;   // halo slices
-    constexpr int ROWS = TT + H;
-    constexpr int RT = TT / SPT;      // thread rows
-    constexpr int NLD = (ROWS + RT - 1) / RT;
+    const int tt = TT > 0 ? TT : (int)blockDim.x * SPT / TX;
+    const int ROWS = tt + H;
+    const int RT = tt / SPT;      // thread rows
+    // loads per thread of the tile and its halo, ceil((TT + H) / RT); a runtime tile needs at
+    // most that of the smallest work-group (64 threads, RT = 64 / TX)
+    constexpr int NLD = TT > 0 ? (TT + H + TT / SPT - 1) / (TT / SPT) : SPT + (H * TX + 63) / 64;
     static_assert(TT % SPT == 0, "slices per thread must divide the tile");
     // preconditioner rows of the tile's TT slices, padded to 16-byte multiples
-    constexpr int CST = (13 * sizeof(R) + 15) / 16 * 16 / sizeof(R);
-    __shared__ cplx<R> ring[PRE ? ROWS * TX : 1];
-    __shared__ __align__(16) R crow[PRE ? TT * CST : 1];
+    constexpr int CST = cgu_cst<R>();
+    __shared__ cplx<R> ring_s[PRE && TT > 0 ? (TT + H) * TX : 1];
+    __shared__ __align__(16) R crow_s[PRE && TT > 0 ? TT * CST : 1];
+    cplx<R>* const ring = TT > 0 ? ring_s : reinterpret_cast<cplx<R>*>(stage);
+    R* const crow = TT > 0 ? crow_s : reinterpret_cast<R*>(stage + (cgu_ring_bytes<R>(tt, TX) + 15u) / 16u * 16u);
     __shared__ double red[64];
-    constexpr int NTHR = TX * TT / SPT;
-    constexpr int NCE = PRE ? (TT * 13 + NTHR - 1) / NTHR : 1;   // coefficients per thread
+    const int NTHR = TX * tt / SPT;
+    constexpr int NCE = PRE ? (13 * SPT + TX - 1) / TX : 1;   // coefficients per thread
     int rho, Vs, Lt, bx, by, blk, nA;
     long long vo, pout, pin;
     const R* ptab = A.ptab;
     if (A.prob != nullptr) {   // ragged batch: one linear grid over every lattice's tiles
         int k = 0;
         for (int j = 1; j < A.nprob; ++j)
-            if (A.prob[j].bwg0 <= (int)blockIdx.x) k = j;
+            if (A.prob[j].bwg0 <= lin) k = j;
         const Prob& P = A.prob[k];
-        rho = k; Vs = P.L * P.L; Lt = P.Lt; blk = blockIdx.x - P.bwg0;
+        rho = k; Vs = P.L * P.L; Lt = P.Lt; blk = lin - P.bwg0;
         bx = blk % P.nsx; by = blk / P.nsx;
         vo = P.voff; pout = P.pb0 + blk; pin = P.pa0; nA = P.nA;
         if (ptab) ptab += P.toff;
     } else {
-        rho = blockIdx.z; Vs = A.Vs; Lt = A.Lt; bx = blockIdx.x; by = blockIdx.y;
-        blk = bx + gridDim.x * by;
-        vo = rho * A.vs; pout = (long long)rho * gridDim.x * gridDim.y + blk; pin = (long long)rho * A.nA; nA = A.nA;
+        rho = bz_; Vs = A.Vs; Lt = A.Lt; bx = bx_; by = by_;
+        blk = bx + gx * by;
+        vo = rho * A.vs; pout = (long long)rho * gx * gy + blk; pin = (long long)rho * A.nA; nA = A.nA;
     }
     const int sx = threadIdx.x % TX, sr = threadIdx.x / TX;
     const int s = bx * TX + sx;
-    const int t0 = by * TT;
+    const int t0 = by * tt;
     const bool col = s < Vs;
     const int sc = col ? s : 0;   // loads of padding columns read site 0 (never stored)
 
@@ -171,7 +193,7 @@ __global__ void __launch_bounds__(TX * TT / SPT) cg_update_kernel(CguArgs<R> A) 
         for (int q = 0; q < NCE; ++q) {
             const int e = threadIdx.x + q * NTHR;
             const int row = e / 13, k = e - row * 13;
-            cv[q] = e < TT * 13 ? ptab[min(t0 + row, Lt - 1) * 13 + k] : R(0);
+            cv[q] = e < tt * 13 ? ptab[min(t0 + row, Lt - 1) * 13 + k] : R(0);
         }
     }
     cplx<R> pp[SPT], xx[SPT], ph[SPT];
@@ -224,7 +246,7 @@ __global__ void __launch_bounds__(TX * TT / SPT) cg_update_kernel(CguArgs<R> A) 
         for (int q = 0; q < NCE; ++q) {
             const int e = threadIdx.x + q * NTHR;
             const int row = e / 13;
-            if (e < TT * 13) crow[row * CST + (e - row * 13)] = cv[q];
+            if (e < tt * 13) crow[row * CST + (e - row * 13)] = cv[q];
         }
         __syncthreads();
     }
@@ -270,15 +292,21 @@ __global__ void __launch_bounds__(TX * TT / SPT) cg_update_kernel(CguArgs<R> A) 
     }
 }
 
+template <typename R, int MODE, bool PRE, int TX, int TT, int SPT>
+__global__ void __launch_bounds__(TX * TT / SPT) cg_update_kernel(CguArgs<R> A) {
+    cgu_body<R, MODE, PRE, TX, TT, SPT>(A, (int)blockIdx.x, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.x,
+                                        (int)gridDim.y);
+}
+
 // Scalars after the last iteration (the reference returns the error of the
 // last iteration and, without graphs, the iteration counter, 828-846).
 // One block per RHS.  If true_res, rr partials hold |b - A x|^2 in FP64 and the
 // error is reported from them even for converged chains.  status_out (optional)
 // gets the chain's kSolve* code.
-__global__ void cg_finalize_kernel(const double* rr_part, int n, const CgState* st_prev, const double* normb, double rtol,
-                                   int true_res, double* err_out, double* iters_out, int* status_out, const Prob* prob) {
+__device__ __forceinline__ void cg_finalize_body(int rho, const double* rr_part, int n, const CgState* st_prev, const double* normb,
+                                                 double rtol, int true_res, double* err_out, double* iters_out, int* status_out,
+                                                 const Prob* prob) {
     __shared__ double red[64];
-    const int rho = blockIdx.x;
     double sums[1];
     long long off = (long long)rho * n;
     if (prob != nullptr) {   // ragged batch: this lattice's update (or, true_res, operator) partials
@@ -308,6 +336,11 @@ __global__ void cg_finalize_kernel(const double* rr_part, int n, const CgState* 
             status_out[rho] = code;
         }
     }
+}
+
+__global__ void cg_finalize_kernel(const double* rr_part, int n, const CgState* st_prev, const double* normb, double rtol,
+                                   int true_res, double* err_out, double* iters_out, int* status_out, const Prob* prob) {
+    cg_finalize_body((int)blockIdx.x, rr_part, n, st_prev, normb, rtol, true_res, err_out, iters_out, status_out, prob);
 }
 
 // Adaptive reliable update of the mixed-precision solver (checked every few

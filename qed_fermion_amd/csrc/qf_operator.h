@@ -112,8 +112,12 @@ constexpr int kMdmPartialPrefetch = 4;
 // register file (2 x 6 patches, complex128):
 //   U = B_tau x_tau;  y_{tau+1} = x_{tau+1} + sM U;  W = B_tau y_{tau+1};
 //   out_tau = y_tau + sD W.
-template <typename R, int BY, int MODE, int NT, bool HOPT, bool PIPE>
-__global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
+// The body takes the work-group's index bid in a launch of nblk work-groups, so
+// a persistent launch can run it for several work items in turn; consecutive
+// calls in one work-group must be separated by a work-group barrier (the
+// exchange buffers are reused).
+template <typename R, int BY, int MODE, bool HOPT, bool PIPE>
+__device__ __forceinline__ void mdagm_body(const MdmArgs<R>& A, int bid, int nblk) {
     extern __shared__ __align__(16) unsigned char qf_smem[];
     cplx<R>* lds = reinterpret_cast<cplx<R>*>(qf_smem);
     __shared__ double red[64];
@@ -126,14 +130,14 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
     if (A.prob != nullptr) {   // ragged batch: find this work-group's lattice
         int k = 0;
         for (int j = 1; j < A.nprob; ++j)
-            if (A.prob[j].wg0 <= (int)blockIdx.x) k = j;
+            if (A.prob[j].wg0 <= bid) k = j;
         const Prob& P = A.prob[k];
-        rho = k; run = blockIdx.x - P.wg0; L = P.L; Lt = P.Lt; T = P.T; nBk = P.nB;
+        rho = k; run = bid - P.wg0; L = P.L; Lt = P.Lt; T = P.T; nBk = P.nB;
         vo = P.voff; pho = P.poff; hoo = P.hoff; pa = P.pa0; pb = P.pb0;
     } else {
-        rho = blockIdx.x % A.nrhs; run = blockIdx.x / A.nrhs; L = A.L; Lt = A.Lt; T = A.T; nBk = A.nB;
+        rho = bid % A.nrhs; run = bid / A.nrhs; L = A.L; Lt = A.Lt; T = A.T; nBk = A.nB;
         vo = rho * A.vs; pho = rho * A.phi_cs; hoo = rho * A.hop_cs;
-        pa = (long long)rho * (gridDim.x / A.nrhs); pb = (long long)rho * A.nB;
+        pa = (long long)rho * (nblk / A.nrhs); pb = (long long)rho * A.nB;
     }
     const int Vs = L * L;
     const int t0 = run * T;
@@ -490,6 +494,11 @@ __global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
         block_sum<1>(v, red);
         if (threadIdx.x == 0) A.dot_part[pa + run] = v[0];
     }
+}
+
+template <typename R, int BY, int MODE, int NT, bool HOPT, bool PIPE>
+__global__ void __launch_bounds__(NT) mdagm_kernel(MdmArgs<R> A) {
+    mdagm_body<R, BY, MODE, HOPT, PIPE>(A, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // Single-sided operators: OP_M (y = Mx), OP_MDAG (y = M^dag x), OP_B

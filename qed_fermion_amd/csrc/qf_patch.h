@@ -624,12 +624,13 @@ __device__ __forceinline__ void store_patch(cplx<R>* __restrict__ s, const cplx<
 // Hop table of one field: hop[tau][y][x][mu] = th_g(x,y,mu) e^{i phi_mu(x,y)},
 // th = tanh(t dtau) for the x-bonds of even sites (group 1), tanh(t dtau/2)
 // otherwise.  Accurate sincos: built once per field, read by every iteration.
+// The body covers entries i0, i0 + stride, ... (a grid-stride loop).
 template <typename R>
-__global__ void hop_table_kernel(const R* __restrict__ phi, long long phi_cs, cplx<R>* __restrict__ hop, int nchain, int L, int Lt,
-                                 double th1, double th2) {
+__device__ __forceinline__ void hop_table_body(const R* __restrict__ phi, long long phi_cs, cplx<R>* __restrict__ hop, int nchain, int L,
+                                               int Lt, double th1, double th2, long long i0, long long stride) {
     const int Vs = L * L;
     const long long n = (long long)nchain * Lt * Vs;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    for (long long i = i0; i < n; i += stride) {
         const int s = (int)(i % Vs);
         const long long q = i / Vs;            // chain * Lt + tau
         const int chain = (int)(q / Lt), tau = (int)(q % Lt);
@@ -648,6 +649,13 @@ __global__ void hop_table_kernel(const R* __restrict__ phi, long long phi_cs, cp
         h[0] = mk<R>(tx * cx, tx * sx);
         h[1] = mk<R>((R)th2 * cy, (R)th2 * sy);
     }
+}
+
+template <typename R>
+__global__ void hop_table_kernel(const R* __restrict__ phi, long long phi_cs, cplx<R>* __restrict__ hop, int nchain, int L, int Lt,
+                                 double th1, double th2) {
+    hop_table_body<R>(phi, phi_cs, hop, nchain, L, Lt, th1, th2, blockIdx.x * (long long)blockDim.x + threadIdx.x,
+                      (long long)gridDim.x * blockDim.x);
 }
 
 // Hop tables of a ragged batch (every lattice's table in one launch; site i of

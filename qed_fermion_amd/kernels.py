@@ -212,7 +212,7 @@ def check_status(status, what="PCG solve", allow_max_iter=True):
     code under graph capture."""
     codes = status.tolist()
     bad = [(i, c) for i, c in enumerate(codes)
-           if c in (_lib.QF_SOLVE_NONFINITE, _lib.QF_SOLVE_BREAKDOWN) or (c == _lib.QF_SOLVE_MAX_ITER and not allow_max_iter)]
+           if c in (_lib.QF_SOLVE_NONFINITE, _lib.QF_SOLVE_BREAKDOWN, _lib.QF_SOLVE_ABORTED) or (c == _lib.QF_SOLVE_MAX_ITER and not allow_max_iter)]
     if bad:
         i, c = bad[0]
         raise QfError(_lib.QF_ERR_INVALID, f"{what}: chain {i}: {_lib.SOLVE_STATUS_NAMES.get(c, c)} "
@@ -245,6 +245,48 @@ def pcg_solve(phi, b, L, Ltau, dtau=0.1, t=1.0, ptab=None, rtol=1e-9, max_iter=4
     st = _status_buf(status, nrhs, dev)
     call("qf_pcg_solve", prec, ptr(phi), _phi_stride(phi, nrhs, N), ptr(b2), ptr(x), nrhs, L, Ltau, float(dtau), float(t),
          ptr(tab), float(rtol), int(max_iter), ptr(work), work.numel(), ptr(err), ptr(iters), ptr(st), stream_handle())
+    return x.view(b.shape), iters, err
+
+
+def pcg_persistent_workspace_bytes(prec, nrhs, L, Ltau):
+    return _lib.lib().qf_pcg_persistent_workspace_bytes(prec, nrhs, L, Ltau)
+
+
+def pcg_solve_persistent(phi, b, L, Ltau, dtau=0.1, t=1.0, ptab=None, rtol=1e-9, max_iter=400, x=None, err=None, iters=None,
+                         status=None):
+    """pcg_solve as ONE persistent launch (qf_pcg_solve_persistent): the same recurrence and
+    outputs for any number of right-hand sides in a single dispatch, leaving as soon as no chain
+    is active.  ptab is required.  Returns (x, iters, err) as device tensors; ``status`` (int32
+    [nrhs], optional) receives each chain's QF_SOLVE_* code (QF_SOLVE_ABORTED if a grid barrier
+    timed out).  Not for graph capture."""
+    N = L * L * Ltau
+    b2 = _vecs(b, N, "b")
+    if ptab is None:
+        raise QfError(_lib.QF_ERR_INVALID, "the persistent solve needs the preconditioner table")
+    _check_dev(phi, ptab)
+    if phi.dtype != _REAL[b.dtype]:
+        raise QfError(_lib.QF_ERR_INVALID, f"phi dtype {phi.dtype} does not match vector dtype {b.dtype}")
+    if ptab.numel() != 13 * Ltau:
+        raise QfError(_lib.QF_ERR_INVALID, f"preconditioner table has {ptab.numel()} entries, expected 13*Ltau={13 * Ltau}")
+    nrhs = b2.shape[0]
+    prec = _prec(b)
+    dev = b.device
+    x = torch.empty_like(b2) if x is None else _buf(x, "x", b.dtype, b2.numel(), dev).view(-1, N)
+    nbytes = pcg_persistent_workspace_bytes(prec, nrhs, L, Ltau)
+    if nbytes == 0:
+        raise QfError(_lib.QF_ERR_UNSUPPORTED, _lib.lib().qf_last_error().decode())
+    # the barrier counters at the end of the workspace start at zero and every solve leaves them zero
+    key = ("persistent", prec, nrhs, L, Ltau, dev)
+    work = _work_cache.get(key)
+    if work is None or work.numel() != nbytes:
+        work = torch.zeros(nbytes, dtype=torch.uint8, device=dev)
+        _work_cache[key] = work
+    err = torch.empty(nrhs, dtype=torch.float64, device=dev) if err is None else _buf(err, "err", torch.float64, nrhs, dev)
+    iters = torch.empty(nrhs, dtype=torch.float64, device=dev) if iters is None else _buf(iters, "iters", torch.float64, nrhs, dev)
+    tab = ptab.to(_REAL[b.dtype]).contiguous()
+    st = _status_buf(status, nrhs, dev)
+    call("qf_pcg_solve_persistent", prec, ptr(phi), _phi_stride(phi, nrhs, N), ptr(b2), ptr(x), nrhs, L, Ltau, float(dtau),
+         float(t), ptr(tab), float(rtol), int(max_iter), ptr(work), work.numel(), ptr(err), ptr(iters), ptr(st), stream_handle())
     return x.view(b.shape), iters, err
 
 

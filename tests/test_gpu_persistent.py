@@ -1,0 +1,148 @@
+"""The persistent multi-RHS PCG (qf_pcg_solve_persistent): the whole solve in ONE launch.
+
+Against the oracle's recurrence (preconditioned_cg_fast_test, hmc_sampler_batch.py:751-846)
+and against the two-launches-per-iteration solve on the same inputs, at small sizes and at the
+batching configuration (128 complex64 Z2 noise vectors, L=48, Ltau=480, one shared field;
+stochastic_estimator.py:427-433), plus the abort path of its bounded grid barrier."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import make_phi, make_vec, relerr
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _k():
+    from qed_fermion_amd import kernels
+    return kernels
+
+
+def _t(a, dev, dtype):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device=dev, dtype=dtype)
+
+
+@pytest.mark.parametrize("L,Lt,kind,shared,max_iter,xtol", [(4, 30, "gauss", False, 60, 1e-10),
+                                                           (6, 60, "uniform", False, 150, 1e-3),
+                                                           (6, 60, "gauss", True, 200, 1e-10),
+                                                           (12, 40, "gauss", False, 200, 1e-10)])
+def test_persistent_f64_follows_oracle(dev, L, Lt, kind, shared, max_iter, xtol):
+    """FP64: the iterates follow the oracle's recurrence (fixed max_iter, early_exit=False
+    semantics), the reported error is the true residual of x, and the iteration counter is the
+    reference's.  The update phase sums its partials over other tiles than the oracle and the
+    two-launch solve, and CG iterates amplify that rounding: 1e-10 on well-conditioned fields,
+    1e-3 after 150 unconverged iterations on the ill-conditioned uniform field."""
+    k = _k()
+    print(f"[persistent f64] L={L} Lt={Lt} {kind} shared={shared}", flush=True)
+    nrhs = 3
+    phi = make_phi(L, Lt, kind, seed=21, nchain=1 if shared else nrhs)
+    R = make_vec(L, Lt, seed=22, nrhs=nrhs)
+    tab = O.precon_table(Lt)
+    b = np.stack([O.apply_Mdag(phi[0 if shared else i], R[i], L, Lt) for i in range(nrhs)])
+    st = torch.empty(nrhs, dtype=torch.int32, device=dev)
+    x, iters, err = k.pcg_solve_persistent(_t(phi.reshape(phi.shape[0], -1), dev, torch.float64), _t(b, dev, torch.complex128), L, Lt,
+                                           ptab=_t(tab, dev, torch.float64), rtol=1e-9, max_iter=max_iter, status=st)
+    x = x.cpu().numpy()
+    for i in range(nrhs):
+        ph = phi[0 if shared else i]
+        xr, it, er = O.pcg(ph, b[i], L, Lt, table=tab, rtol=1e-9, max_iter=max_iter, early_exit=False)
+        assert relerr(x[i], xr) < xtol, relerr(x[i], xr)
+        true_res = relerr(O.apply_MdagM(ph, x[i], L, Lt), b[i])
+        assert abs(err[i].item() - true_res) < 1e-2 * true_res + 1e-13, (err[i].item(), true_res)
+        assert int(iters[i].item()) == it
+        assert st[i].item() == (0 if er <= 1e-9 else 1)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_persistent_equals_two_launch_solve(dev, prec):
+    """Same inputs through qf_pcg_solve (two launches per iteration, max_iter iterations) and the
+    persistent launch (which leaves once every chain has stopped): equal iteration counts and
+    status codes, solutions equal to the rounding of the partial-sum order, and the early exit
+    leaves nothing undone (chains converge at different iterations, far below max_iter)."""
+    k = _k()
+    L, Lt, nrhs = 12, 120, 5
+    print(f"[persistent vs two-launch] {prec}", flush=True)
+    rd, cd, rtol, tol = (torch.float64, torch.complex128, 1e-10, 1e-8) if prec == "f64" else (torch.float32, torch.complex64, 1e-5, 2e-4)
+    phi = make_phi(L, Lt, "gauss", seed=31, nchain=nrhs)
+    phi[1] *= 0.2   # a better-conditioned chain converges earlier
+    b = make_vec(L, Lt, seed=32, nrhs=nrhs)
+    ptab = k.precon_table(Lt, device="cuda", dtype=rd)
+    pd, bd = _t(phi.reshape(nrhs, -1), dev, rd), _t(b, dev, cd)
+    s1 = torch.empty(nrhs, dtype=torch.int32, device=dev)
+    s2 = torch.empty(nrhs, dtype=torch.int32, device=dev)
+    x1, it1, e1 = k.pcg_solve(pd, bd, L, Lt, ptab=ptab, rtol=rtol, max_iter=2000, status=s1)
+    x2, it2, e2 = k.pcg_solve_persistent(pd, bd, L, Lt, ptab=ptab, rtol=rtol, max_iter=2000, status=s2)
+    torch.cuda.synchronize()
+    print(f"[persistent vs two-launch] iters {it1.tolist()} / {it2.tolist()}", flush=True)
+    assert s1.tolist() == s2.tolist() == [0] * nrhs
+    assert max(abs(a - c) for a, c in zip(it1.tolist(), it2.tolist())) <= 1
+    assert max(it2.tolist()) < 1999 and min(it2.tolist()) < max(it2.tolist())
+    assert e2.max().item() <= rtol
+    x1n, x2n = x1.cpu().numpy(), x2.cpu().numpy()
+    for i in range(nrhs):
+        assert relerr(x2n[i], x1n[i]) < tol, (i, relerr(x2n[i], x1n[i]))
+        # FP32: the iterated residual drifts from the true one (recomputed here in FP64)
+        res = relerr(O.apply_MdagM(phi[i], x2n[i].astype(np.complex128), L, Lt), b[i])
+        res1 = relerr(O.apply_MdagM(phi[i], x1n[i].astype(np.complex128), L, Lt), b[i])
+        assert res <= 3 * rtol if prec == "f64" else res <= 2 * res1 + rtol, (i, res, res1)
+
+
+def test_persistent_multi_rhs_c4(dev):
+    """The batching configuration in one launch: 128 complex64 Z2 noise vectors at L=48,
+    Ltau=480 on one shared per-bond Gaussian field, PCG to 1e-5.  Every chain converges; the
+    oracle recomputes the true residual of the first and last solutions; the iteration counts
+    match the two-launch solve."""
+    k = _k()
+    L, Lt, nrhs = 48, 480, 128
+    print("[persistent C4]", flush=True)
+    N = L * L * Lt
+    phi = make_phi(L, Lt, "gauss", seed=4801)
+    rng = np.random.default_rng(4802)
+    eta = (((rng.integers(0, 2, (nrhs, N)) * 2 - 1) + 1j * (rng.integers(0, 2, (nrhs, N)) * 2 - 1)) / np.sqrt(2))
+    phid = _t(phi.reshape(1, -1), dev, torch.float32)
+    ptab = k.precon_table(Lt, device="cuda", dtype=torch.float32)
+    b = k.apply("Mdag", phid, _t(eta, dev, torch.complex64), L, Lt)
+    st = torch.empty(nrhs, dtype=torch.int32, device=dev)
+    x, it, err = k.pcg_solve_persistent(phid, b, L, Lt, ptab=ptab, rtol=1e-5, max_iter=1500, status=st)
+    torch.cuda.synchronize()
+    print(f"[persistent C4] iterations {it.min().item():.0f}..{it.max().item():.0f}, max err {err.max().item():.2e}", flush=True)
+    assert err.max().item() <= 1e-5 and st.tolist() == [0] * nrhs
+    pick = [0, nrhs - 1]
+    x2, it2, _ = k.pcg_solve(phid, b[pick].contiguous(), L, Lt, ptab=ptab, rtol=1e-5, max_iter=1500)
+    assert max(abs(it[i].item() - it2[j].item()) for j, i in enumerate(pick)) <= 1
+    xn, bn = x.cpu().numpy(), b.cpu().numpy()
+    for i in pick:
+        res = relerr(O.apply_MdagM(phi[0], xn[i].astype(np.complex128), L, Lt), bn[i].astype(np.complex128))
+        assert res < 3e-5, (i, res)
+
+
+def test_persistent_abort_path(dev):
+    """A zero wait limit makes every grid barrier time out: every work-group leaves, the launch
+    ends normally, every chain reports QF_SOLVE_ABORTED with a NaN error, and the barrier
+    counters are left at zero, so the next solve on the same workspace runs normally."""
+    k = _k()
+    from qed_fermion_amd import _lib
+    L, Lt, nrhs = 12, 120, 4
+    print("[persistent abort path]", flush=True)
+    phi = make_phi(L, Lt, "gauss", seed=41, nchain=nrhs)
+    b = make_vec(L, Lt, seed=42, nrhs=nrhs)
+    ptab = k.precon_table(Lt, device="cuda", dtype=torch.float64)
+    pd, bd = _t(phi.reshape(nrhs, -1), dev, torch.float64), _t(b, dev, torch.complex128)
+    st = torch.empty(nrhs, dtype=torch.int32, device=dev)
+    old = _lib.lib().qf_set_persistent_timeout_us(0)
+    try:
+        _, _, err = k.pcg_solve_persistent(pd, bd, L, Lt, ptab=ptab, rtol=1e-10, max_iter=500, status=st)
+        torch.cuda.synchronize()
+    finally:
+        _lib.lib().qf_set_persistent_timeout_us(old)
+    assert st.tolist() == [_lib.QF_SOLVE_ABORTED] * nrhs
+    assert torch.isnan(err).all().item()
+    with pytest.raises(_lib.QfError):
+        k.check_status(st)
+    x, it, err = k.pcg_solve_persistent(pd, bd, L, Lt, ptab=ptab, rtol=1e-10, max_iter=500, status=st)
+    torch.cuda.synchronize()
+    assert st.tolist() == [0] * nrhs and err.max().item() <= 1e-10
+    x1, it1, _ = k.pcg_solve(pd, bd, L, Lt, ptab=ptab, rtol=1e-10, max_iter=500)
+    assert it.tolist() == it1.tolist()
+    assert relerr(x.cpu().numpy(), x1.cpu().numpy()) < 1e-9
