@@ -156,9 +156,9 @@ int qf_pcg_solve_noalloc(int prec, const void* phi, long long phi_cs, const void
  * iters_out, status_out and x equal those of qf_pcg_solve up to the order of
  * the partial sums).  ptab must not be NULL.  Every barrier wait is bounded
  * (10 s): a timed-out solve marks every chain QF_SOLVE_ABORTED.  work: at
- * least qf_pcg_persistent_workspace_bytes(prec, nrhs, L, Ltau) bytes, whose
- * last 256 bytes (barrier counters) must be zero before the first call on
- * that workspace (every call leaves them zero).  Calls sharing a workspace
+ * least n = qf_pcg_persistent_workspace_bytes(prec, nrhs, L, Ltau) bytes; the
+ * 256 bytes at offset n - 256 (barrier counters) must be zero before the
+ * first call on that workspace (every call leaves them zero).  Calls sharing a workspace
  * must not overlap.  L up to 64 (work-groups of at most 512 threads); returns
  * QF_ERR_UNSUPPORTED otherwise.  Not for stream capture. */
 size_t qf_pcg_persistent_workspace_bytes(int prec, int nrhs, int L, int Ltau);
@@ -201,6 +201,20 @@ size_t qf_pcg_ragged_workspace_bytes(int prec, const void* plan_host);
 int qf_pcg_solve_ragged(int prec, const void* plan_host, const void* plan_dev, const void* phi, const void* b, void* x, double dtau,
                         double t, const void* ptab, double rtol, int max_iter, void* work, size_t work_bytes, double* err_out,
                         double* iters_out, int* status_out, void* stream);
+
+/* qf_pcg_solve_ragged as ONE persistent cooperative launch (the persistent
+ * solve of qf_pcg_solve_persistent over the plan's work items): same outputs,
+ * one dispatch, leaving as soon as every lattice has stopped.  ptab (the
+ * concatenated tables) must not be NULL; the plan's largest lattice may need
+ * at most 512 threads (L up to 44).  work: at least
+ * n = qf_pcg_ragged_persistent_workspace_bytes(prec, plan_host) bytes; the 256
+ * bytes at offset n - 256 (barrier counters) must be zero before the first
+ * call on that workspace (every call leaves them zero).  Not for stream
+ * capture. */
+size_t qf_pcg_ragged_persistent_workspace_bytes(int prec, const void* plan_host);
+int qf_pcg_solve_ragged_persistent(int prec, const void* plan_host, const void* plan_dev, const void* phi, const void* b, void* x,
+                                   double dtau, double t, const void* ptab, double rtol, int max_iter, void* work, size_t work_bytes,
+                                   double* err_out, double* iters_out, int* status_out, void* stream);
 
 /* The mixed-precision solver of qf_pcg_mixed_solve over a ragged batch: complex64
  * iterations with FP64 dot products and adaptive FP64 reliable updates (at most every

@@ -86,6 +86,8 @@ SIGNATURES = {
     "qf_apply_ragged": (_I, [_I, _I, _P, _P, _P, _P, _P, _D, _D, _P]),
     "qf_pcg_ragged_workspace_bytes": (_SZ, [_I, _P]),
     "qf_pcg_solve_ragged": (_I, [_I, _P, _P, _P, _P, _P, _D, _D, _P, _D, _I, _P, _SZ, _P, _P, _P, _P]),
+    "qf_pcg_ragged_persistent_workspace_bytes": (_SZ, [_I, _P]),
+    "qf_pcg_solve_ragged_persistent": (_I, [_I, _P, _P, _P, _P, _P, _D, _D, _P, _D, _I, _P, _SZ, _P, _P, _P, _P]),
     "qf_pcg_mixed_ragged_workspace_bytes": (_SZ, [_P]),
     "qf_pcg_mixed_solve_ragged": (_I, [_P, _P, _P, _P, _P, _P, _D, _D, _P, _D, _I, _I, _P, _SZ, _P, _P, _P, _P]),
     "qf_fermion_force": (_I, [_I, _P, _LL, _P, _P, _I, _I, _I, _D, _D, _P]),

@@ -566,7 +566,8 @@ int launch_pers(PcgPersArgs<R>& a, int blk, hipStream_t s) {
     constexpr bool PIPE = mdm_pipe<R, BY>();
     constexpr bool HOPT = cg_uses_hop<R>(BY);
     auto k = pcg_persistent_kernel<R, BY, HOPT, PIPE>;
-    const size_t lds = std::max<size_t>(lds_bytes_c<R>(BY, blk), cgu_stage_bytes<R>(blk * kPersSPT / kPersTX, kPersTX));
+    if (kPersTX * a.u.TB / kPersSPT > blk) return fail(QF_ERR_INVALID, "persistent solve: update tile larger than the work-group");
+    const size_t lds = std::max<size_t>(lds_bytes_c<R>(BY, blk), cgu_stage_bytes<R>(a.u.TB, kPersTX));
     int rc = set_lds(k, lds);
     if (rc) return rc;
     int dev = 0, ncu = 0, per = 0, khz = 0;
@@ -575,7 +576,7 @@ int launch_pers(PcgPersArgs<R>& a, int blk, hipStream_t s) {
     QF_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
     QF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, blk, lds));
     if (per < 1) return fail(QF_ERR_UNSUPPORTED, "persistent solve: no work-group of %d threads fits a CU", blk);
-    const long long items = std::max((long long)a.nrhs * a.nA, (long long)a.nrhs * a.nsx * a.ntile);
+    const long long items = a.prob ? std::max(a.nAi, a.nBi) : std::max((long long)a.nrhs * a.nA, (long long)a.nrhs * a.nsx * a.ntile);
     const int G = (int)std::max(1LL, std::min<long long>((long long)per * ncu, items));
     a.timeout_ticks = g_pers_timeout_us.load() * (long long)khz / 1000LL;   // wall-clock ticks per wait
     void* args[] = {&a};
@@ -1310,6 +1311,37 @@ int pcg_ragged(const RaggedHead* h, const Prob* pd, const R* phi, const cplx<R>*
     QF_LAUNCHED();
     return QF_OK;
 }
+
+// The ragged PCG as one persistent launch: the plan's operator and update work items,
+// work-groups of at least 256 threads (the 16 x 64 update tiles of the plan, four slices per
+// thread), x updated every iteration as in pcg_ragged.
+template <typename R>
+int pcg_ragged_persistent(const RaggedHead* h, const Prob* pd, const R* phi, const cplx<R>* b, cplx<R>* x, double dtau, double t,
+                          const R* ptab, double rtol, int max_iter, const RgWork& w, void* bar, double* err_out, double* iters_out,
+                          int* status_out, hipStream_t s) {
+    const int np = h->nprob;
+    const int blk = std::max(h->nthreads, kPersTX * h->tb / kPersSPT);
+    if (blk > 512) return fail(QF_ERR_UNSUPPORTED, "persistent ragged solve: work-groups of %d threads (at most 512: L up to 44)", blk);
+    PcgPersArgs<R> a{};
+    a.u.ptab = ptab; a.u.prob = pd; a.u.nprob = np; a.u.TB = h->tb;
+    a.m.phi = phi; a.m.hop = (const cplx<R>*)w.HOP; a.m.hp = make_hop<R>(dtau, t); a.m.prob = pd; a.m.nprob = np; a.m.nrhs = 1;
+    a.m.normb = w.normb; a.m.rtol = rtol; a.m.dot_part = w.pap; a.m.z = (const cplx<R>*)w.Z; a.m.y = (cplx<R>*)w.OP;
+    a.nrhs = np; a.max_iter = max_iter; a.paired = 0;
+    a.b = b; a.x = x;
+    a.R2[0] = (cplx<R>*)w.R[0]; a.R2[1] = (cplx<R>*)w.R[1]; a.P2[0] = (cplx<R>*)w.P[0]; a.P2[1] = (cplx<R>*)w.P[1];
+    a.Z = (cplx<R>*)w.Z; a.OP = (cplx<R>*)w.OP;
+    a.rzp = w.rzp; a.rrp = w.rrp; a.pap = w.pap; a.ps = h->nB_tot;
+    a.st = w.st; a.normb = w.normb; a.brk = w.brk;
+    if constexpr (sizeof(R) == 8) {
+        a.phi = phi; a.hop = (cplx<R>*)w.HOP;
+        a.th1 = std::tanh(t * dtau); a.th2 = std::tanh(0.5 * t * dtau);
+    }
+    a.rtol = rtol; a.err_out = err_out; a.iters_out = iters_out; a.status_out = status_out;
+    unsigned* c = static_cast<unsigned*>(bar);
+    a.gb.count = c; a.gb.done = c + 16; a.gb.abort = c + 32;
+    a.prob = pd; a.nAi = h->nwg_op; a.nBi = h->nwg_upd; a.nsites = h->nsites;
+    return launch_pers<R, 2>(a, blk, s);
+}
 }  // namespace
 
 extern "C" {
@@ -1346,6 +1378,34 @@ int qf_pcg_solve_ragged(int prec, const void* plan_host, const void* plan_dev, c
                                  err_out, iters_out, status_out, s);
     return pcg_ragged<double>(h, pd, (const double*)phi, (const double2*)b, (double2*)x, dtau, t, (const double*)ptab, rtol, max_iter, w,
                               err_out, iters_out, status_out, s);
+}
+
+size_t qf_pcg_ragged_persistent_workspace_bytes(int prec, const void* plan_host) {
+    const size_t n = qf_pcg_ragged_workspace_bytes(prec, plan_host);
+    return n ? n + kPersBarBytes : 0;
+}
+
+int qf_pcg_solve_ragged_persistent(int prec, const void* plan_host, const void* plan_dev, const void* phi, const void* b, void* x,
+                                   double dtau, double t, const void* ptab, double rtol, int max_iter, void* work, size_t work_bytes,
+                                   double* err_out, double* iters_out, int* status_out, void* stream) {
+    const RaggedHead* h;
+    const Prob* pd;
+    int rc = ragged_head(plan_host, plan_dev, &h, &pd);
+    if (rc) return rc;
+    if (!phi || !b || !x || !work || !err_out || !iters_out) return fail(QF_ERR_INVALID, "null pointer");
+    if (!ptab) return fail(QF_ERR_INVALID, "the persistent solve needs the preconditioner tables");
+    if (max_iter < 1 || max_iter > 1000000) return fail(QF_ERR_INVALID, "max_iter=%d out of range [1, 1000000]", max_iter);
+    if (prec != QF_PREC_F32 && prec != QF_PREC_F64) return fail(QF_ERR_INVALID, "prec %d", prec);
+    const RgWork w = carve_ragged(work, prec, false, h);
+    if (work_bytes < w.bytes + kPersBarBytes)
+        return fail(QF_ERR_INVALID, "workspace of %zu bytes, need %zu", work_bytes, w.bytes + kPersBarBytes);
+    void* bar = static_cast<char*>(work) + w.bytes;
+    hipStream_t s = (hipStream_t)stream;
+    if (prec == QF_PREC_F32)
+        return pcg_ragged_persistent<float>(h, pd, (const float*)phi, (const float2*)b, (float2*)x, dtau, t, (const float*)ptab, rtol,
+                                            max_iter, w, bar, err_out, iters_out, status_out, s);
+    return pcg_ragged_persistent<double>(h, pd, (const double*)phi, (const double2*)b, (double2*)x, dtau, t, (const double*)ptab, rtol,
+                                         max_iter, w, bar, err_out, iters_out, status_out, s);
 }
 
 int qf_pcg_mixed_solve_ragged(const void* plan_host, const void* plan_dev, const double* phi64, const float* phi32, const void* b, void* x,

@@ -67,10 +67,11 @@ struct CguArgs {
 // LDS, so no thread marches in tau and all loads of a block are in flight at
 // once.  Slices outside [0, Lt) of the window wrap cyclically (P is cyclic).
 // Partial sums: one (rz, rr) pair per block, blk = bx + nsx * by.
-// TT = 0: the tile height follows the work-group, TT = blockDim.x * SPT / TX
-// (at most 512 threads), for a persistent launch whose work-group size is set
-// by the operator phase; the staged residual and band rows then live in the
-// caller's LDS at stage (cgu_stage_bytes).  The body takes the tile
+// TT = 0: the tile height is A.TB, set at run time for a persistent launch
+// whose work-group size is set by the operator phase (TX * A.TB / SPT threads
+// work, at most blockDim.x; the rest only join the barriers); the staged
+// residual and band rows then live in the caller's LDS at stage
+// (cgu_stage_bytes).  The body takes the tile
 // coordinates (bx, by) of rhs bz in a grid of gx x gy tiles per rhs (lin:
 // linear tile index of a ragged batch); consecutive calls in one work-group
 // must be separated by a work-group barrier.
@@ -87,7 +88,7 @@ template <typename R, int MODE, bool PRE, int TX, int TT, int SPT>
 __device__ __forceinline__ void cgu_body(const CguArgs<R>& A, int lin, int bx_, int by_, int bz_, int gx, int gy,
                                          unsigned char* stage = nullptr) {
     constexpr int H = PRE ? 12 : 0;   // halo slices
-    const int tt = TT > 0 ? TT : (int)blockDim.x * SPT / TX;
+    const int tt = TT > 0 ? TT : A.TB;
     const int ROWS = tt + H;
     const int RT = tt / SPT;      // thread rows
     // loads per thread of the tile and its halo, ceil((TT + H) / RT); a runtime tile needs at
@@ -102,6 +103,7 @@ __device__ __forceinline__ void cgu_body(const CguArgs<R>& A, int lin, int bx_, 
     R* const crow = TT > 0 ? crow_s : reinterpret_cast<R*>(stage + (cgu_ring_bytes<R>(tt, TX) + 15u) / 16u * 16u);
     __shared__ double red[64];
     const int NTHR = TX * tt / SPT;
+    const bool live = TT > 0 || (int)threadIdx.x < NTHR;   // runtime tiles may leave threads idle
     constexpr int NCE = PRE ? (13 * SPT + TX - 1) / TX : 1;   // coefficients per thread
     int rho, Vs, Lt, bx, by, blk, nA;
     long long vo, pout, pin;
@@ -128,13 +130,14 @@ __device__ __forceinline__ void cgu_body(const CguArgs<R>& A, int lin, int bx_, 
 
     if constexpr (MODE == CGU_ITER) {
         if (A.st[rho].act == 0.0) {   // converged chain: x and r stay frozen (uniform)
-            const R ah = (R)A.xalpha[rho];
-            // stopped right after a held iteration (ah = 0: the held step broke down, x stays)
-            if (A.xmode == 2 && A.st_held[rho].act != 0.0 && ah != R(0)) {
+            // stopped right after a held iteration (ah = 0: the held step broke down, x stays);
+            // xalpha and st_held are only read in paired mode
+            const R ah = (A.xmode == 2 && A.st_held[rho].act != 0.0) ? (R)A.xalpha[rho] : R(0);
+            if (ah != R(0)) {
 #pragma unroll
                 for (int j = 0; j < SPT; ++j) {
                     const int tau = t0 + sr + j * RT;
-                    if (col && tau < Lt) {
+                    if (live && col && tau < Lt) {
                         const long long io = vo + (long long)tau * Vs + s;
                         const cplx<R> xv = A.x[io], pv = A.pprev[io];
                         A.x[io] = mk<R>(xv.x + ah * pv.x, xv.y + ah * pv.y);
@@ -240,13 +243,13 @@ __device__ __forceinline__ void cgu_body(const CguArgs<R>& A, int lin, int bx_, 
 #pragma unroll
         for (int k = 0; k < NLD; ++k) {
             const int h = sr + k * RT;
-            if (h < ROWS) ring[h * TX + sx] = rw[k];
+            if (live && h < ROWS) ring[h * TX + sx] = rw[k];
         }
 #pragma unroll
         for (int q = 0; q < NCE; ++q) {
             const int e = threadIdx.x + q * NTHR;
             const int row = e / 13;
-            if (e < tt * 13) crow[row * CST + (e - row * 13)] = cv[q];
+            if (live && e < tt * 13) crow[row * CST + (e - row * 13)] = cv[q];
         }
         __syncthreads();
     }
@@ -270,7 +273,7 @@ __device__ __forceinline__ void cgu_body(const CguArgs<R>& A, int lin, int bx_, 
             r = rw[j];
             zz = r;
         }
-        if (col && tau < Lt) {
+        if (live && col && tau < Lt) {
             const long long io = vo + (long long)tau * Vs + s;
             if constexpr (MODE == CGU_ITER) {
                 if (A.xmode != 1) A.x[io] = brk ? xx[j] : mk<R>(xx[j].x + alpha * pp[j].x, xx[j].y + alpha * pp[j].y);

@@ -661,9 +661,9 @@ __global__ void hop_table_kernel(const R* __restrict__ phi, long long phi_cs, cp
 // Hop tables of a ragged batch (every lattice's table in one launch; site i of
 // the concatenated vectors belongs to the last lattice with voff <= i).
 template <typename R>
-__global__ void hop_table_ragged_kernel(const R* __restrict__ phi, cplx<R>* __restrict__ hop, const Prob* __restrict__ prob, int nprob,
-                                        long long nsites, double th1, double th2) {
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nsites; i += (long long)gridDim.x * blockDim.x) {
+__device__ __forceinline__ void hop_table_ragged_body(const R* __restrict__ phi, cplx<R>* __restrict__ hop, const Prob* __restrict__ prob,
+                                                      int nprob, long long nsites, double th1, double th2, long long i0, long long stride) {
+    for (long long i = i0; i < nsites; i += stride) {
         int k = 0;
         for (int j = 1; j < nprob; ++j)
             if (prob[j].voff <= i) k = j;
@@ -686,6 +686,13 @@ __global__ void hop_table_ragged_kernel(const R* __restrict__ phi, cplx<R>* __re
         h[0] = mk<R>(tx * cx, tx * sx);
         h[1] = mk<R>((R)th2 * cy, (R)th2 * sy);
     }
+}
+
+template <typename R>
+__global__ void hop_table_ragged_kernel(const R* __restrict__ phi, cplx<R>* __restrict__ hop, const Prob* __restrict__ prob, int nprob,
+                                        long long nsites, double th1, double th2) {
+    hop_table_ragged_body<R>(phi, hop, prob, nprob, nsites, th1, th2, blockIdx.x * (long long)blockDim.x + threadIdx.x,
+                             (long long)gridDim.x * blockDim.x);
 }
 
 }  // namespace qf

@@ -92,9 +92,15 @@ struct PcgPersArgs {
     int* status_out;
     GridBar gb;
     long long timeout_ticks;
+    // ragged batch (prob != nullptr): independent lattices, one chain each, with the
+    // operator and update work items of the plan (nrhs = lattices)
+    const Prob* prob;
+    int nAi, nBi;
+    long long nsites;
 };
 
-// Update tiles of the persistent solve: 16 sites x (blockDim / 4) slices, four per thread.
+// Update tiles of the persistent solve: 16 sites x P.u.TB slices, four per thread (uniform
+// lattices: TB = blockDim / 4; a ragged plan's TB = 64 with 256 working threads).
 constexpr int kPersTX = 16, kPersSPT = 4;
 
 template <typename R, int BY, bool HOPT, bool PIPE>
@@ -104,22 +110,27 @@ __global__ void __launch_bounds__(512) pcg_persistent_kernel(PcgPersArgs<R> P) {
     extern __shared__ __align__(16) unsigned char qf_smem[];
     const int G = gridDim.x;
     const int nrhs = P.nrhs;
-    const int nAi = nrhs * P.nA, nBt = P.nsx * P.ntile, nBi = nrhs * nBt;
+    const bool rag = P.prob != nullptr;
+    const int nBt = P.nsx * P.ntile;
+    const int nAi = rag ? P.nAi : nrhs * P.nA, nBi = rag ? P.nBi : nrhs * nBt;
     unsigned epoch = 0;
     bool ok = true;
     int last = 1;   // parity slot of the last completed iteration (1: the start)
 
     if constexpr (HOPT) {
-        hop_table_body<R>(P.phi, P.phi_cs, P.hop, P.nchain, P.m.L, P.m.Lt, P.th1, P.th2, blockIdx.x * (long long)blockDim.x + threadIdx.x,
-                          (long long)G * blockDim.x);
+        const long long i0 = blockIdx.x * (long long)blockDim.x + threadIdx.x, stride = (long long)G * blockDim.x;
+        if (rag)
+            hop_table_ragged_body<R>(P.phi, P.hop, P.prob, nrhs, P.nsites, P.th1, P.th2, i0, stride);
+        else
+            hop_table_body<R>(P.phi, P.phi_cs, P.hop, P.nchain, P.m.L, P.m.Lt, P.th1, P.th2, i0, stride);
     }
     {   // start: x = 0, r = b, z = P b, partials into slot 1, state slot 1
         CguArgs<R> u = P.u;
         u.rold = P.b; u.x = P.x; u.rnew = P.R2[1]; u.z = P.Z;
         u.st_init = P.st + nrhs; u.rz_part = P.rzp + P.ps; u.rr_part = P.rrp + P.ps; u.brk = P.brk;
         for (int it = blockIdx.x; it < nBi; it += G) {
-            const int rho = it / nBt, k = it - rho * nBt;
-            cgu_body<R, CGU_INIT, true, kPersTX, 0, kPersSPT>(u, it, k % P.nsx, k / P.nsx, rho, P.nsx, P.ntile, qf_smem);
+            const int rho = rag ? 0 : it / nBt, k = rag ? 0 : it - rho * nBt;   // a ragged tile is found from it
+            cgu_body<R, CGU_INIT, true, kPersTX, 0, kPersSPT>(u, it, rag ? 0 : k % P.nsx, rag ? 0 : k / P.nsx, rho, P.nsx, P.ntile, qf_smem);
             __syncthreads();
         }
     }
@@ -144,8 +155,8 @@ __global__ void __launch_bounds__(512) pcg_persistent_kernel(PcgPersArgs<R> P) {
         u.pap = P.pap; u.st = P.st + cur * nrhs; u.rz_part = P.rzp + cur * P.ps; u.rr_part = P.rrp + cur * P.ps;
         u.ru_flag = nullptr; u.xmode = xmode; u.pprev = P.P2[prv]; u.xalpha = P.xal; u.st_held = P.st + prv * nrhs; u.brk = P.brk;
         for (int it = blockIdx.x; it < nBi; it += G) {
-            const int rho = it / nBt, k = it - rho * nBt;
-            cgu_body<R, CGU_ITER, true, kPersTX, 0, kPersSPT>(u, it, k % P.nsx, k / P.nsx, rho, P.nsx, P.ntile, qf_smem);
+            const int rho = rag ? 0 : it / nBt, k = rag ? 0 : it - rho * nBt;   // a ragged tile is found from it
+            cgu_body<R, CGU_ITER, true, kPersTX, 0, kPersSPT>(u, it, rag ? 0 : k % P.nsx, rag ? 0 : k / P.nsx, rho, P.nsx, P.ntile, qf_smem);
             __syncthreads();
         }
         if (!(ok = grid_barrier(P.gb, epoch, P.timeout_ticks))) break;
@@ -159,7 +170,7 @@ __global__ void __launch_bounds__(512) pcg_persistent_kernel(PcgPersArgs<R> P) {
     if (ok) {
         for (int rho = blockIdx.x; rho < nrhs; rho += G) {
             cg_finalize_body(rho, P.rrp + last * P.ps, nBt, P.st + last * nrhs, P.normb, P.rtol, 0, P.err_out, P.iters_out, P.status_out,
-                             nullptr);
+                             P.prob);
             __syncthreads();
         }
     }

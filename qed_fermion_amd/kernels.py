@@ -489,6 +489,38 @@ def pcg_solve_ragged(plan, phi, b, dtau=0.1, t=1.0, ptab=None, rtol=1e-9, max_it
     return x, iters, err
 
 
+def pcg_solve_ragged_persistent(plan, phi, b, dtau=0.1, t=1.0, ptab=None, rtol=1e-9, max_iter=400, x=None, status=None):
+    """pcg_solve_ragged as ONE persistent launch (qf_pcg_solve_ragged_persistent), leaving as
+    soon as every lattice has stopped.  ptab (the plan's concatenated tables) is required.
+    Returns (x, iters, err) with one iters / err entry per lattice; ``status`` (int32
+    [lattices], optional) gets the QF_SOLVE_* codes.  Not for graph capture."""
+    _check_dev(phi, b, ptab)
+    if ptab is None:
+        raise QfError(_lib.QF_ERR_INVALID, "the persistent solve needs the preconditioner tables")
+    if b.numel() != plan.nsites or phi.numel() != 2 * plan.nsites:
+        raise QfError(_lib.QF_ERR_INVALID, "ragged vector / field sizes do not match the plan")
+    if b.dtype not in _REAL or phi.dtype != _REAL[b.dtype]:
+        raise QfError(_lib.QF_ERR_INVALID, f"ragged dtypes: b {b.dtype}, phi {phi.dtype}")
+    if ptab.numel() != 13 * sum(plan.Ltaus):
+        raise QfError(_lib.QF_ERR_INVALID, "preconditioner tables do not match the plan")
+    prec = _prec(b)
+    nbytes = _lib.lib().qf_pcg_ragged_persistent_workspace_bytes(prec, plan.host.ctypes.data)
+    key = ("ragged_persistent", prec, tuple(plan.Ls), tuple(plan.Ltaus), b.device)
+    work = _work_cache.get(key)
+    if work is None or work.numel() != nbytes:   # barrier counters start at zero; every solve leaves them zero
+        work = torch.zeros(nbytes, dtype=torch.uint8, device=b.device)
+        _work_cache[key] = work
+    x = torch.empty_like(b) if x is None else _buf(x, "x", b.dtype, b.numel(), b.device)
+    n = len(plan.Ls)
+    err = torch.empty(n, dtype=torch.float64, device=b.device)
+    iters = torch.empty(n, dtype=torch.float64, device=b.device)
+    tab = ptab.to(_REAL[b.dtype]).contiguous()
+    st = _status_buf(status, n, b.device)
+    call("qf_pcg_solve_ragged_persistent", prec, plan.host.ctypes.data, ptr(plan.dev), ptr(phi), ptr(b), ptr(x), float(dtau),
+         float(t), ptr(tab), float(rtol), int(max_iter), ptr(work), work.numel(), ptr(err), ptr(iters), ptr(st), stream_handle())
+    return x, iters, err
+
+
 def pcg_mixed_solve_ragged(plan, phi64, b, dtau=0.1, t=1.0, ptab=None, rtol=1e-10, max_iter=400, ru_period=50, phi32=None,
                            x=None, status=None):
     """Mixed-precision PCG over a ragged batch (complex64 iterations, FP64 reliable updates at most

@@ -146,3 +146,39 @@ def test_persistent_abort_path(dev):
     x1, it1, _ = k.pcg_solve(pd, bd, L, Lt, ptab=ptab, rtol=1e-10, max_iter=500)
     assert it.tolist() == it1.tolist()
     assert relerr(x.cpu().numpy(), x1.cpu().numpy()) < 1e-9
+
+
+LS = [12, 18, 24, 30, 12, 18, 24, 30]
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_persistent_ragged_set(dev, prec):
+    """The batching configuration's ragged set (L in {12, 18, 24, 30} twice, Ltau = 10 L, each
+    lattice its own field) in one persistent launch: the same iterations (within one) and
+    solutions as the two-launch ragged solve, and each lattice's true residual recomputed by the
+    oracle."""
+    from qed_fermion_amd import kernels as K
+    print(f"[persistent ragged] {prec}", flush=True)
+    rd, cd, rtol, tol = (torch.float64, torch.complex128, 1e-10, 1e-8) if prec == "f64" else (torch.float32, torch.complex64, 1e-5, 2e-4)
+    plan = K.RaggedPlan(LS, [10 * L for L in LS])
+    phis = [make_phi(L, 10 * L, "gauss", seed=61 + k).reshape(-1) for k, L in enumerate(LS)]
+    vecs = [make_vec(L, 10 * L, seed=161 + k).reshape(-1) for k, L in enumerate(LS)]
+    phi = torch.from_numpy(np.concatenate(phis)).to(dev, rd)
+    b = torch.from_numpy(np.concatenate(vecs)).to(dev, cd)
+    ptab = plan.precon_table(dtype=rd, device=dev)
+    s1 = torch.empty(len(LS), dtype=torch.int32, device=dev)
+    s2 = torch.empty(len(LS), dtype=torch.int32, device=dev)
+    x1, it1, e1 = K.pcg_solve_ragged(plan, phi, b, ptab=ptab, rtol=rtol, max_iter=3000, status=s1)
+    x2, it2, e2 = K.pcg_solve_ragged_persistent(plan, phi, b, ptab=ptab, rtol=rtol, max_iter=3000, status=s2)
+    torch.cuda.synchronize()
+    print(f"[persistent ragged] iters {it1.tolist()} / {it2.tolist()}", flush=True)
+    assert s1.tolist() == s2.tolist() == [0] * len(LS)
+    assert max(abs(a - c) for a, c in zip(it1.tolist(), it2.tolist())) <= 1
+    assert e2.max().item() <= rtol
+    xs1, xs2 = plan.split(x1), plan.split(x2)
+    for k, L in enumerate(LS):
+        a, c = xs2[k].cpu().numpy(), xs1[k].cpu().numpy()
+        assert relerr(a, c) < tol, (k, relerr(a, c))
+        res = relerr(O.apply_MdagM(phis[k].reshape(10 * L, L, L, 2), a.astype(np.complex128), L, 10 * L), vecs[k])
+        res1 = relerr(O.apply_MdagM(phis[k].reshape(10 * L, L, L, 2), c.astype(np.complex128), L, 10 * L), vecs[k])
+        assert res <= 3 * rtol if prec == "f64" else res <= 2 * res1 + rtol, (k, res, res1)
