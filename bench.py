@@ -85,11 +85,12 @@ def parse():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-c2", action="store_true", help="time the full C2 trajectory (L=24, 40 leapfrog) on the CPU instead of its first 4 steps")
     ap.add_argument("--no-tune", action="store_true", help="use the built-in tilings instead of the autotuner")
-    ap.add_argument("--probe", choices=["per_chain", "per_chain_hop", "shared", "bond_bond", "pcg"], default=None,
+    ap.add_argument("--probe", choices=["per_chain", "per_chain_hop", "shared", "bond_bond", "pcg", "persistent"], default=None,
                     help="per_chain (from phi) / per_chain_hop / shared: a calibration copy, then the roofline M^dag M "
                          "three times per tiling (for rocprofv3 --pmc); bond_bond: the bond-bond estimators of one chain "
                          "at L=36, Ltau=360, Nrv=40, twice; pcg: a calibration copy, then 20 iterations of the headline "
-                         "mixed PCG (--bs chains, own fields, no chain stops)")
+                         "mixed PCG (--bs chains, own fields, no chain stops); persistent: the L=48 128-RHS and ragged "
+                         "solves as two launches per iteration and as one persistent launch each")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batching (L=48, 128 Z2 RHS; ragged set), bond-bond and L=66 measurements")
     ap.add_argument("--seed", type=int, default=2024)
@@ -421,10 +422,13 @@ def extras(torch, K, args, dev):
     x = torch.empty_like(b)
     it = args.ref_max_iter
     t_se = timed(torch, lambda: K.pcg_solve(phi, b, L, Lt, ptab=ptab, rtol=0.0, max_iter=it, x=x))
+    t_sep = timed(torch, lambda: K.pcg_solve_persistent(phi, b, L, Lt, ptab=ptab, rtol=0.0, max_iter=it, x=x))
     hop = K.hop_table(phi, L, Lt)
     t_op = timed(torch, lambda: K.apply_hop("MdagM", hop, eta, L, Lt, out=x), reps=5)
     out["se_L48_128rhs"] = {"solve_s": t_se, "iterations": it, "ms_per_iteration": t_se / it * 1e3,
-                            "mdagm_GBps": (nrv * 16 * N + 16 * N) / t_op / 1e9, "mdagm_ms": t_op * 1e3}
+                            "mdagm_GBps": (nrv * 16 * N + 16 * N) / t_op / 1e9, "mdagm_ms": t_op * 1e3,
+                            "persistent_solve_s": t_sep,
+                            "note": "solve_s: two launches per iteration; persistent_solve_s: the whole solve as one launch"}
     log(f"[bench] SE L=48 128 RHS: {t_se:.3f} s per {it}-iteration solve, M^dag M {out['se_L48_128rhs']['mdagm_GBps']:.0f} GB/s")
     del eta, b, x, hop
     # -- bond-bond estimators of one chain, L=36, Ltau=360, Nrv=40 (the reference's Nrv at this size)
@@ -449,6 +453,7 @@ def extras(torch, K, args, dev):
     br = torch.randn(plan.nsites, generator=g, dtype=torch.complex128).to(dev, torch.complex64)
     ptr_ = plan.precon_table(dtype=torch.float32, device=dev)
     t_rg = timed(torch, lambda: K.pcg_solve_ragged(plan, phr, br, ptab=ptr_, rtol=0.0, max_iter=it))
+    t_rgp = timed(torch, lambda: K.pcg_solve_ragged_persistent(plan, phr, br, ptab=ptr_, rtol=0.0, max_iter=it))
     t_rop = timed(torch, lambda: K.apply_ragged(plan, phr, br), reps=10)
     phr64 = phr.double()
     br64 = br.to(torch.complex128)
@@ -456,7 +461,8 @@ def extras(torch, K, args, dev):
                                                           ru_period=args.ru_period, phi32=phr))
     out["ragged_8_lattices"] = {"L": Ls, "solve_s": t_rg, "iterations": it, "ms_per_iteration": t_rg / it * 1e3,
                                 "mdagm_ms": t_rop * 1e3, "mdagm_GBps": 24 * plan.nsites / t_rop / 1e9,
-                                "mixed_solve_s": t_rgm, "mixed_ms_per_iteration": t_rgm / it * 1e3}
+                                "mixed_solve_s": t_rgm, "mixed_ms_per_iteration": t_rgm / it * 1e3,
+                                "persistent_solve_s": t_rgp}
     log(f"[bench] ragged batch: {t_rg:.3f} s per {it}-iteration solve ({t_rgm:.3f} s mixed), one M^dag M launch "
         f"{t_rop * 1e3:.3f} ms")
     # -- differentiation at full size: L=66, Ltau=660, a 20-step mixed trajectory
@@ -619,6 +625,39 @@ def probe_pcg(args):
                       "copy_bytes_each_way": xc.numel() * 4, "tiling": K.get_tiling(0, bs, L, Lt)}), flush=True)
 
 
+def probe_persistent(args):
+    """The batching configuration's solves, each as two launches per iteration and as one
+    persistent launch, 400 iterations at rtol 0 (no chain stops): 128 complex64 Z2 noise vectors
+    at L=48, Ltau=480 on one field, and the ragged set L in {12, 18, 24, 30} x 2.  Under
+    rocprofv3 --kernel-trace each persistent solve is one dispatch."""
+    import torch
+    from qed_fermion_amd import kernels as K
+    dev = torch.device("cuda:0")
+    it = 400
+    L, Lt, nrv = 48, 480, 128
+    N = L * L * Lt
+    g = torch.Generator(device="cpu").manual_seed(48)
+    phi = ((torch.rand(1, 2 * N, generator=g, dtype=torch.float64) * 2 - 1) * math.pi).to(dev, torch.float32)
+    eta = (torch.randint(0, 2, (nrv, N), generator=g) * 2 - 1).to(torch.float32)
+    eta = (torch.complex(eta, (torch.randint(0, 2, (nrv, N), generator=g) * 2 - 1).to(torch.float32)) / math.sqrt(2)).to(dev)
+    b = K.apply("Mdag", phi, eta, L, Lt)
+    ptab = K.precon_table(Lt, dtype=torch.float32, device=dev)
+    x = torch.empty_like(b)
+    t_two = timed(torch, lambda: K.pcg_solve(phi, b, L, Lt, ptab=ptab, rtol=0.0, max_iter=it, x=x))
+    t_per = timed(torch, lambda: K.pcg_solve_persistent(phi, b, L, Lt, ptab=ptab, rtol=0.0, max_iter=it, x=x))
+    del eta, b, x
+    Ls = [12, 18, 24, 30, 12, 18, 24, 30]
+    plan = K.RaggedPlan(Ls, [10 * v for v in Ls], device=dev)
+    phr = ((torch.rand(2 * plan.nsites, generator=g, dtype=torch.float64) * 2 - 1) * math.pi).to(dev, torch.float32)
+    br = torch.randn(plan.nsites, generator=g, dtype=torch.complex128).to(dev, torch.complex64)
+    ptr_ = plan.precon_table(dtype=torch.float32, device=dev)
+    r_two = timed(torch, lambda: K.pcg_solve_ragged(plan, phr, br, ptab=ptr_, rtol=0.0, max_iter=it))
+    r_per = timed(torch, lambda: K.pcg_solve_ragged_persistent(plan, phr, br, ptab=ptr_, rtol=0.0, max_iter=it))
+    print(json.dumps({"probe": "persistent", "iterations": it,
+                      "se_L48_128rhs_s": {"two_launch": t_two, "persistent": t_per},
+                      "ragged_8_lattices_s": {"two_launch": r_two, "persistent": r_per}}), flush=True)
+
+
 def probe_bond_bond():
     """The bond-bond estimators of one chain (L=36, Ltau=360, Nrv=40, complex64, synthetic
     G_eta), evaluated twice; prints the time of the second evaluation."""
@@ -694,6 +733,9 @@ def main():
         return
     if args.probe == "pcg":
         probe_pcg(args)
+        return
+    if args.probe == "persistent":
+        probe_persistent(args)
         return
     if args.probe:
         probe(args)
