@@ -120,10 +120,42 @@ int pick_by(int L, int prec, int want = 0) {
     return pick_by_default(L, prec);
 }
 
+// Lattices too wide for a whole slice per work-group (no patch shape fits) are
+// split into bands of rows: 2x2 patches, up to 1024 threads, each work-group
+// computing its band plus `hal` halo rows on both sides (4 rows for one B
+// application, 8 for M^dag M and the force, whose factor sweeps reach twice
+// as far), with nband bands of nown rows each (the last one shorter).
+struct BandCfg {
+    int nband = 1, H = 0, hal = 0, nown = 0, nthr = 0;
+};
+inline bool square_ok(int L) { return pick_by_default(L, 0) != 0; }
+constexpr int kHalB = 4, kHalMdm = 8;
+
+BandCfg band_cfg(int L, int hal) {
+    BandCfg c;
+    const int half = L / 2;
+    const int rows_max = 2 * (1024 / half);          // band rows a 1024-thread work-group holds
+    const int own_max = rows_max - 2 * hal;
+    if (own_max < 2) return c;                        // nband = 1, nthr = 0: does not fit
+    c.nband = (L + own_max - 1) / own_max;
+    c.nown = 2 * ((L + 2 * c.nband - 1) / (2 * c.nband));
+    c.nband = (L + c.nown - 1) / c.nown;
+    c.hal = hal;
+    c.H = c.nown + 2 * hal;
+    c.nthr = half * (c.H / 2);
+    return c;
+}
+
+template <typename A>
+void set_band(A& a, const BandCfg& c) {
+    a.nband = c.nband; a.H = c.H; a.hal = c.hal; a.nown = c.nown;
+}
+
 Tiling tiling(int prec, int nrhs, int L, int Lt) {
     Tiling t;
     const Tuning tu = find_tuning(prec, L, Lt, nrhs);
-    t.by = pick_by(L, prec, tu.by);
+    const bool band = !square_ok(L);
+    t.by = band ? 2 : pick_by(L, prec, tu.by);
     const long long units = (long long)nrhs * Lt;
     // tau slices per operator work-group: about 2048 work-groups in flight
     // over the batch, at least 4 slices so the per-run prologue (one extra B
@@ -131,6 +163,7 @@ Tiling tiling(int prec, int nrhs, int L, int Lt) {
     t.T = tu.t ? tu.t : (int)std::max(4LL, std::min(16LL, units / 2048));
     t.T = std::min(t.T, Lt);
     t.nA = (Lt + t.T - 1) / t.T;
+    if (band) t.nA *= band_cfg(L, kHalMdm).nband;   // one partial sum per (run, band)
     const int Vs = L * L;
     // update kernel tile: TB tau slices x cgu_tx(TB) sites; the 12-slice
     // preconditioner halo costs 12 / TB extra reads of r and M^dag M p
@@ -167,7 +200,8 @@ int check_lattice(int L, int Lt, int nrhs) {
     if (L < 2 || (L & 1)) return fail(QF_ERR_ODD_L, "L=%d: the checkerboard bond groups need an even L >= 2", L);
     if (Lt < 1) return fail(QF_ERR_INVALID, "Ltau=%d must be >= 1", Lt);
     if (nrhs < 1 || nrhs > 65535) return fail(QF_ERR_INVALID, "nrhs=%d out of range [1, 65535]", nrhs);
-    if (pick_by_default(L, 0) == 0) return fail(QF_ERR_UNSUPPORTED, "L=%d: no patch shape fits a 512/1024-thread work-group", L);
+    if (!square_ok(L) && band_cfg(L, kHalMdm).nthr == 0)
+        return fail(QF_ERR_UNSUPPORTED, "L=%d: no band of 2x2 patches with its halo fits a 1024-thread work-group (L <= 226)", L);
     return QF_OK;
 }
 
@@ -202,7 +236,15 @@ int launch_mdm_k(const MdmArgs<R>& a0, int nrhs, hipStream_t s) {
     MdmArgs<R> a = a0;
     a.nrhs = nrhs;
     constexpr bool PIPE = mdm_pipe<R, BY>();
-    const dim3 grid((unsigned)(((a.Lt + a.T - 1) / a.T) * nrhs)), blk = blk_of(a.L, BY);
+    dim3 blk = blk_of(a.L, BY);
+    a.nband = 1;
+    if (!square_ok(a.L)) {   // bands of 2x2 patches (BY = 2 from the tiling)
+        if (BY != 2) return fail(QF_ERR_UNSUPPORTED, "L=%d: bands use 2x2 patches", a.L);
+        const BandCfg c = band_cfg(a.L, kHalMdm);
+        set_band(a, c);
+        blk = dim3((unsigned)(((c.nthr + 63) / 64) * 64));
+    }
+    const dim3 grid((unsigned)(((a.Lt + a.T - 1) / a.T) * a.nband * nrhs));
     const size_t lds = lds_bytes_c<R>(BY, (int)blk.x);
     if (blk.x <= 512) {
         auto k = mdagm_kernel<R, BY, MODE, 512, HOPT, PIPE>;
@@ -251,9 +293,19 @@ int launch_mdm(const MdmArgs<R>& a, int by, int nrhs, hipStream_t s) {
 }
 
 template <typename R, int BY, int OP>
-int launch_op_by(const OpArgs<R>& a, int nrhs, hipStream_t s) {
-    const size_t lds = lds_bytes<R>(a.L, 1);
-    const dim3 grid((a.Lt + a.T - 1) / a.T, nrhs), blk = blk_of(a.L, BY);
+int launch_op_by(const OpArgs<R>& a0, int nrhs, hipStream_t s) {
+    OpArgs<R> a = a0;
+    dim3 blk = blk_of(a.L, BY);
+    BandCfg c;
+    c.H = a.L;
+    if (!square_ok(a.L)) {
+        if (BY != 2) return fail(QF_ERR_UNSUPPORTED, "L=%d: bands use 2x2 patches", a.L);
+        c = band_cfg(a.L, kHalB);
+        blk = dim3((unsigned)(((c.nthr + 63) / 64) * 64));
+    }
+    set_band(a, c);
+    const size_t lds = lds_bytes<R>(a.L, 1, c.H);
+    const dim3 grid((a.Lt + a.T - 1) / a.T, nrhs, c.nband);
     if (blk.x <= 512) {
         auto k = op_kernel<R, BY, OP, 512>;
         int rc = set_lds(k, lds);
@@ -280,9 +332,19 @@ int launch_op(const OpArgs<R>& a, int by, int nrhs, hipStream_t s) {
 }
 
 template <typename R, int BY>
-int launch_force_by(const ForceArgs<R>& a, int nrhs, hipStream_t s) {
-    const size_t lds = lds_bytes<R>(a.L, 2);
-    const dim3 grid((a.Lt + a.T - 1) / a.T, nrhs), blk = blk_of(a.L, BY);
+int launch_force_by(const ForceArgs<R>& a0, int nrhs, hipStream_t s) {
+    ForceArgs<R> a = a0;
+    dim3 blk = blk_of(a.L, BY);
+    BandCfg c;
+    c.H = a.L;
+    if (!square_ok(a.L)) {
+        if (BY != 2) return fail(QF_ERR_UNSUPPORTED, "L=%d: bands use 2x2 patches", a.L);
+        c = band_cfg(a.L, kHalMdm);
+        blk = dim3((unsigned)(((c.nthr + 63) / 64) * 64));
+    }
+    set_band(a, c);
+    const size_t lds = lds_bytes<R>(a.L, 2, c.H);
+    const dim3 grid((a.Lt + a.T - 1) / a.T, nrhs, c.nband);
     if (blk.x <= 512) {
         auto k = force_kernel<R, BY, 512>;
         int rc = set_lds(k, lds);
@@ -759,7 +821,7 @@ int qf_set_tuning(int prec, int by, int t, int tb) {
 
 int qf_set_tuning_shape(int prec, int L, int Ltau, int nrhs, int by, int t, int tb) {
     if (prec != QF_PREC_F32 && prec != QF_PREC_F64) return fail(QF_ERR_INVALID, "prec %d", prec);
-    if (by != 0 && !by_ok(L, by)) return fail(QF_ERR_INVALID, "patch height %d does not fit L=%d", by, L);
+    if (by != 0 && !by_ok(L, by) && !(by == 2 && !square_ok(L))) return fail(QF_ERR_INVALID, "patch height %d does not fit L=%d", by, L);
     if (t < 0 || (tb != 0 && !tb_ok(tb)) || nrhs < 0 || L < 2 || Ltau < 1) return fail(QF_ERR_INVALID, "bad shape or tile size");
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (auto& e : g_shape_tune)
@@ -1022,7 +1084,7 @@ size_t qf_pcg_persistent_workspace_bytes(int prec, int nrhs, int L, int Ltau) {
         return 0;
     }
     const PersPlan pl = pers_plan(prec, nrhs, L, Ltau);
-    if (pl.blk > 512) {
+    if (pl.blk > 512 || !square_ok(L)) {
         fail(QF_ERR_UNSUPPORTED, "L=%d: the persistent solve takes work-groups of at most 512 threads", L);
         return 0;
     }
@@ -1040,7 +1102,7 @@ int qf_pcg_solve_persistent(int prec, const void* phi, long long phi_cs, const v
     if (max_iter < 1 || max_iter > 1000000) return fail(QF_ERR_INVALID, "max_iter=%d out of range [1, 1000000]", max_iter);
     if (prec != QF_PREC_F32 && prec != QF_PREC_F64) return fail(QF_ERR_INVALID, "prec %d", prec);
     const PersPlan pl = pers_plan(prec, nrhs, L, Ltau);
-    if (pl.blk > 512) return fail(QF_ERR_UNSUPPORTED, "L=%d: the persistent solve takes work-groups of at most 512 threads", L);
+    if (pl.blk > 512 || !square_ok(L)) return fail(QF_ERR_UNSUPPORTED, "L=%d: the persistent solve takes work-groups of at most 512 threads", L);
     const Tiling t64 = tiling(QF_PREC_F64, nrhs, L, Ltau);
     const CgWork w = carve(work, prec, 0, nrhs, L, Ltau, pl.t, t64);
     if (work_bytes < w.bytes + kPersBarBytes)

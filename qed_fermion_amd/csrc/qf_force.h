@@ -82,6 +82,7 @@ struct ForceArgs {
     long long f_cs;
     int L, Lt, T;
     Hop<R> hp;
+    int nband, H, hal, nown;   // bands (blockIdx.z), as in MdmArgs
 };
 
 template <typename R, int BY, int NT>
@@ -92,7 +93,7 @@ __global__ void __launch_bounds__(NT) force_kernel(ForceArgs<R> A) {
     const int L = A.L, Lt = A.Lt, Vs = L * L;
     const int t0 = blockIdx.x * A.T;
     const int t1 = min(t0 + A.T, Lt);
-    const Geo g = make_geo<BY>(L, threadIdx.x);
+    const Geo g = make_geo<BY>(L, threadIdx.x, band_of(L, A.nband, A.H, A.hal, A.nown, blockIdx.z));
     const R* phi = A.phi + rho * A.phi_cs;
     const cplx<R>* xi = A.xi + rho * A.vs;
     R* Ft = A.Ft + rho * A.f_cs;
@@ -122,12 +123,12 @@ __global__ void __launch_bounds__(NT) force_kernel(ForceArgs<R> A) {
         force_factor<R, BY, 2>(Av, Wv, F, cf, A.hp, lds, g, phase);
         force_factor<R, BY, 3>(Av, Wv, F, cf, A.hp, lds, g, phase);
         force_factor<R, BY, 4>(Av, Wv, F, cf, A.hp, lds, g, phase);
-        if (g.valid) {
+        if (g.own) {
             cplx<R>* fo = reinterpret_cast<cplx<R>*>(Ft + (long long)tau * 2 * Vs);
 #pragma unroll
             for (int r = 0; r < BY; ++r) {
-                fo[(g.y0 + r) * L + 2 * g.a] = mk<R>(F[2 * r][0], F[2 * r][1]);
-                fo[(g.y0 + r) * L + 2 * g.a + 1] = mk<R>(F[2 * r + 1][0], F[2 * r + 1][1]);
+                fo[(g.gy0 + r) * L + 2 * g.a] = mk<R>(F[2 * r][0], F[2 * r][1]);
+                fo[(g.gy0 + r) * L + 2 * g.a + 1] = mk<R>(F[2 * r + 1][0], F[2 * r + 1][1]);
             }
         }
 #pragma unroll

@@ -70,7 +70,18 @@ struct MdmArgs {
     // MODE 2 (true residual in FP64): r = b - M^dag M x written as complex64
     const cplx<R>* b;
     float2* r_lo;
+    // bands (nband > 1): each slice is split into nband bands of nown rows, each
+    // work-group computing band rows plus hal halo rows on both sides (H rows in all)
+    int nband, H, hal, nown;
 };
+
+// The band of work-group band index k (the whole lattice when nband <= 1): its own
+// rows are [k * nown, min((k + 1) * nown, L)), with hal rows of halo on both sides.
+__host__ __device__ inline RowBand band_of(int L, int nband, int H, int hal, int nown, int k) {
+    if (nband <= 1) return RowBand{0, L, 0, L};
+    const int r0 = k * nown - hal;
+    return RowBand{r0 < 0 ? r0 + L : r0, H, hal, min(nown, L - k * nown)};
+}
 
 enum { MDM_PLAIN = 0, MDM_CG = 1, MDM_RESID = 2 };
 
@@ -125,24 +136,29 @@ __device__ __forceinline__ void mdagm_body(const MdmArgs<R>& A, int bid, int nbl
     // 1-D grid, right-hand side fastest: the work-groups dispatched together
     // (round-robin over the XCDs) march the same tau run, so a shared field's
     // coefficients are fetched into each XCD's L2 once for all of them.
-    int rho, run, L, Lt, T, nBk;
+    int rho, run, rest, L, Lt, T, nBk;
     long long vo, pho, hoo, pa, pb;
+    RowBand bd;
     if (A.prob != nullptr) {   // ragged batch: find this work-group's lattice
         int k = 0;
         for (int j = 1; j < A.nprob; ++j)
             if (A.prob[j].wg0 <= bid) k = j;
         const Prob& P = A.prob[k];
-        rho = k; run = bid - P.wg0; L = P.L; Lt = P.Lt; T = P.T; nBk = P.nB;
+        rho = k; run = bid - P.wg0; rest = run; L = P.L; Lt = P.Lt; T = P.T; nBk = P.nB;
+        bd = RowBand{0, L, 0, L};
         vo = P.voff; pho = P.poff; hoo = P.hoff; pa = P.pa0; pb = P.pb0;
     } else {
-        rho = bid % A.nrhs; run = bid / A.nrhs; L = A.L; Lt = A.Lt; T = A.T; nBk = A.nB;
+        // per rhs the work-groups are (run, band), band fastest
+        const int nbd = A.nband > 1 ? A.nband : 1;
+        rho = bid % A.nrhs; rest = bid / A.nrhs; run = rest / nbd; L = A.L; Lt = A.Lt; T = A.T; nBk = A.nB;
+        bd = band_of(L, A.nband, A.H, A.hal, A.nown, rest - run * nbd);
         vo = rho * A.vs; pho = rho * A.phi_cs; hoo = rho * A.hop_cs;
         pa = (long long)rho * (nblk / A.nrhs); pb = (long long)rho * A.nB;
     }
     const int Vs = L * L;
     const int t0 = run * T;
     const int t1 = min(t0 + T, Lt);
-    const Geo g = make_geo<BY>(L, threadIdx.x);
+    const Geo g = make_geo<BY>(L, threadIdx.x, bd);
     const PatchOff po = patch_off<R>(g);
     const XAddr xa = make_xaddr<BY>(g);
     const unsigned vbytes = (unsigned)Vs * Lt * S;   // one vector
@@ -183,7 +199,7 @@ __device__ __forceinline__ void mdagm_body(const MdmArgs<R>& A, int bid, int nbl
         // a chain that stopped in an earlier iteration carries its state and loads nothing
         const CgState S0 = A.st_prev[rho];
         if (S0.act == 0.0) {   // uniform over the block
-            if (run == 0 && threadIdx.x == 0) A.st_out[rho] = S0;
+            if (rest == 0 && threadIdx.x == 0) A.st_out[rho] = S0;
             return;
         }
     }
@@ -269,7 +285,7 @@ __device__ __forceinline__ void mdagm_body(const MdmArgs<R>& A, int bid, int nbl
                 if (act != 0.0) it += 1.0;
             }
         }
-        if (run == 0 && threadIdx.x == 0) {
+        if (rest == 0 && threadIdx.x == 0) {
             CgState o; o.rz = rzv; o.act = act; o.err = err; o.iters = it; o.stat = stat;
             A.st_out[rho] = o;
             if (A.iter == 0) A.normb[rho] = nb;
@@ -295,7 +311,7 @@ __device__ __forceinline__ void mdagm_body(const MdmArgs<R>& A, int bid, int nbl
     auto emit = [&](int tau, const cplx<R> (&bb)[2 * BY], const cplx<R> (&Wv)[2 * BY]) {
         cplx<R> Z[2 * BY];
         axpy_patch<R, BY>(Z, Y, sgnD(tau, Lt) * cn, Wv);
-        if (g.valid) {
+        if (g.own) {
             if constexpr (MODE == MDM_RESID) {
 #pragma unroll
                 for (int k = 0; k < 2 * BY; ++k) {
@@ -303,7 +319,7 @@ __device__ __forceinline__ void mdagm_body(const MdmArgs<R>& A, int bid, int nbl
                     dsum += (double)rx_ * rx_ + (double)ry_ * ry_;
                     Z[k] = mk<R>(rx_, ry_);
                 }
-                float2* rp = r_lo + (long long)tau * Vs + g.y0 * L + 2 * g.a;
+                float2* rp = r_lo + (long long)tau * Vs + g.gy0 * L + 2 * g.a;
 #pragma unroll
                 for (int r = 0; r < BY; ++r)
                     *reinterpret_cast<float4*>(rp + r * L) =
@@ -329,7 +345,7 @@ __device__ __forceinline__ void mdagm_body(const MdmArgs<R>& A, int bid, int nbl
         make(U, rw);   // x_{t0-1}
         make(X, rx);   // x_{t0}
         if constexpr (MODE == MDM_CG) {
-            if (g.valid) store_patch_g<R, BY>(X, pnew + (long long)t0 * Vs, po, L);
+            if (g.own) store_patch_g<R, BY>(X, pnew + (long long)t0 * Vs, po, L);
         }
         touch_lines<NTOUCH>(td, rs_c, (unsigned)t0n * cslice, cslice, nlanes);
         apply_Bc<R, BY>(U, c2, lds, xa, phase);                 // B_{t0-1} x_{t0-1}
@@ -339,8 +355,6 @@ __device__ __forceinline__ void mdagm_body(const MdmArgs<R>& A, int bid, int nbl
             const bool more = tau + 1 < t1;
             const int tn = (tau + 1 == Lt) ? 0 : tau + 1;
             const int tnn = (tn + 1 == Lt) ? 0 : tn + 1;
-            cplx<R> bb[2 * BY];
-            if constexpr (MODE == MDM_RESID) load_patch_buf<R, BY>(bb, rs_b, po, (unsigned)tau * sstride, L);
             ldraw(rx, tn);                                      // x_{tau+1} (the halo slice when !more)
             if constexpr (ONESET) {
                 if (tau != t0) coef(cA, tau);                   // touched one step ahead: an L2 hit
@@ -356,12 +370,24 @@ __device__ __forceinline__ void mdagm_body(const MdmArgs<R>& A, int bid, int nbl
             apply_Bc<R, BY>(U, cA, lds, xa, phase);             // B_tau x_tau
             make(X, rx);                                        // x_{tau+1}
             if constexpr (MODE == MDM_CG) {
-                if (more && g.valid) store_patch_g<R, BY>(X, pnew + (long long)tn * Vs, po, L);
+                if (more && g.own) store_patch_g<R, BY>(X, pnew + (long long)tn * Vs, po, L);
             }
             axpy_patch<R, BY>(W, X, sgnM(tn) * cn, U);          // y_{tau+1}
 #pragma unroll
             for (int k = 0; k < 2 * BY; ++k) Yn[k] = W[k];
-            if (more) load_patch_buf<R, BY>(X, rs_re, po, (unsigned)tn * sstride, L);   // x_{tau+1} again
+            if (more) {   // x_{tau+1} again
+                if (MODE == MDM_CG && A.nband > 1) {
+                    // a band's halo rows of p are stored by the neighbouring bands: recompute them
+                    Raw rr;
+                    ldraw(rr, tn);
+                    make(X, rr);
+                } else {
+                    load_patch_buf<R, BY>(X, rs_re, po, (unsigned)tn * sstride, L);
+                }
+            }
+            // b of the residual: loaded late, held only across the last factors
+            cplx<R> bb[2 * BY];
+            if constexpr (MODE == MDM_RESID) load_patch_buf<R, BY>(bb, rs_b, po, (unsigned)tau * sstride, L);
             apply_Bc<R, BY>(W, cA, lds, xa, phase);             // B_tau y_{tau+1}
             emit(tau, bb, W);
 #pragma unroll
@@ -381,7 +407,7 @@ __device__ __forceinline__ void mdagm_body(const MdmArgs<R>& A, int bid, int nbl
     make(U, rw);   // x_{t0-1}
     make(X, rx);   // x_{t0}
     if constexpr (MODE == MDM_CG) {
-        if (g.valid) store_patch_g<R, BY>(X, pnew + (long long)t0 * Vs, po, L);
+        if (g.own) store_patch_g<R, BY>(X, pnew + (long long)t0 * Vs, po, L);
     }
 #pragma unroll
     for (int k = 0; k < 2 * BY; ++k) W[k] = X[k];
@@ -412,7 +438,7 @@ __device__ __forceinline__ void mdagm_body(const MdmArgs<R>& A, int bid, int nbl
 #pragma unroll
             for (int k = 0; k < 2 * BY; ++k) U[k] = X[k];
             if constexpr (MODE == MDM_CG) {
-                if (g.valid) store_patch_g<R, BY>(X, pnew + (long long)wrap(tau + 1) * Vs, po, L);
+                if (g.own) store_patch_g<R, BY>(X, pnew + (long long)wrap(tau + 1) * Vs, po, L);
             }
             coefc_from_angles<R, BY>(cB, qr[kc], A.hp);        // C_{tau+1}
             ldphi(qr[kc], clampA(tau + 1 + PFD));              // angles of slice tau+1+PFD
@@ -449,7 +475,7 @@ __device__ __forceinline__ void mdagm_body(const MdmArgs<R>& A, int bid, int nbl
 #pragma unroll
             for (int k = 0; k < 2 * BY; ++k) U[k] = X[k];
             if constexpr (MODE == MDM_CG) {
-                if (g.valid) store_patch_g<R, BY>(X, pnew + (long long)tn * Vs, po, L);
+                if (g.own) store_patch_g<R, BY>(X, pnew + (long long)tn * Vs, po, L);
             }
             if constexpr (REGPF) {
                 if (tau + 2 < t1) coef(cC, tnn);
@@ -492,7 +518,7 @@ __device__ __forceinline__ void mdagm_body(const MdmArgs<R>& A, int bid, int nbl
     if (A.dot_part != nullptr) {
         double v[1] = {dsum};
         block_sum<1>(v, red);
-        if (threadIdx.x == 0) A.dot_part[pa + run] = v[0];
+        if (threadIdx.x == 0) A.dot_part[pa + rest] = v[0];
     }
 }
 
@@ -519,6 +545,7 @@ struct OpArgs {
     Hop<R> hp;
     int nfac;   // OP_B: number of leading factors of B_tau applied (7 = all)
     R cpart;    // OP_B: product of cosh(a_g) over those factors (cn for all 7)
+    int nband, H, hal, nown;   // bands (blockIdx.z), as in MdmArgs
 };
 
 template <typename R, int BY, int OP, int NT>
@@ -529,7 +556,7 @@ __global__ void __launch_bounds__(NT) op_kernel(OpArgs<R> A) {
     const int L = A.L, Lt = A.Lt, Vs = L * L;
     const int t0 = blockIdx.x * A.T;
     const int t1 = min(t0 + A.T, Lt);
-    const Geo g = make_geo<BY>(L, threadIdx.x);
+    const Geo g = make_geo<BY>(L, threadIdx.x, band_of(L, A.nband, A.H, A.hal, A.nown, blockIdx.z));
     const R* phi = A.phi + rho * A.phi_cs;
     const cplx<R>* x = A.x + rho * A.vs;
     cplx<R>* y = A.y + rho * A.vs;
@@ -554,7 +581,7 @@ __global__ void __launch_bounds__(NT) op_kernel(OpArgs<R> A) {
             coef(cf, tp);
             apply_B<R, BY>(U, cf, A.hp, lds, g, phase);        // U = B_{tau-1} x_{tau-1}
             load_patch<R, BY>(X, x + (long long)tau * Vs, g);
-            if (g.valid) {
+            if (g.own) {
                 cplx<R> Z[2 * BY];
                 axpy_patch<R, BY>(Z, X, sgnM(tau) * A.hp.cn, U);
                 store_patch<R, BY>(y + (long long)tau * Vs, Z, g);
@@ -573,7 +600,7 @@ __global__ void __launch_bounds__(NT) op_kernel(OpArgs<R> A) {
 #pragma unroll
             for (int k = 0; k < 2 * BY; ++k) Xn[k] = U[k];
             apply_B<R, BY>(U, cf, A.hp, lds, g, phase);        // U = B_tau x_{tau+1}
-            if (g.valid) {
+            if (g.own) {
                 cplx<R> Z[2 * BY];
                 axpy_patch<R, BY>(Z, X, sgnD(tau, Lt) * A.hp.cn, U);
                 store_patch<R, BY>(y + (long long)tau * Vs, Z, g);
@@ -591,7 +618,7 @@ __global__ void __launch_bounds__(NT) op_kernel(OpArgs<R> A) {
                 apply_B_partial<R, BY>(U, A.nfac, cf, A.hp, lds, g, phase);
 #pragma unroll
             for (int k = 0; k < 2 * BY; ++k) U[k] = mk<R>(A.cpart * U[k].x, A.cpart * U[k].y);
-            if (g.valid) store_patch<R, BY>(y + (long long)tau * Vs, U, g);
+            if (g.own) store_patch<R, BY>(y + (long long)tau * Vs, U, g);
         }
     }
 }

@@ -23,20 +23,35 @@ namespace qf {
 // Factor order of B_tau, applied right to left (first entry applied first).
 __host__ __device__ constexpr int seq_group(int k) { return (k == 3) ? 1 : (k == 2 || k == 4) ? 2 : (k == 1 || k == 5) ? 3 : 4; }
 
+// The work-group's region is H rows of the lattice (all L, or a band), L sites
+// wide, periodic in x; its rows wrap within the region, so for a band the
+// partners of its edge rows are wrong and the values they reach are only kept
+// where the band's own rows lie far enough inside (the halo, see RowBand).
 struct Geo {
-    int L;       // lattice extent (even)
-    int a, b;    // patch coordinates: a in [0, L/2), b in [0, L/BY)
-    int nb;      // L / BY
-    int y0;      // b * BY
+    int L;       // lattice extent (even): row length and x period
+    int H;       // rows of the region (L, or a band with its halo)
+    int a, b;    // patch coordinates: a in [0, L/2), b in [0, H/BY)
+    int nb;      // H / BY
+    int y0;      // b * BY, row in the region
+    int gy0;     // the lattice row of y0
     bool valid;  // false for padding lanes of the last wave
+    bool own;    // valid and in the region's own rows: stores and partial sums
+};
+
+// A band: region rows [row0, row0 + H) of the lattice (mod L), of which rows
+// [hal, hal + nown) are its own output (the rest is halo, recomputed by the
+// neighbouring bands).  The whole lattice is row0 = 0, H = L, hal = 0, nown = L.
+struct RowBand {
+    int row0, H, hal, nown;
 };
 
 template <int BY>
-__device__ __forceinline__ Geo make_geo(int L, int tid) {
+__device__ __forceinline__ Geo make_geo(int L, int tid, const RowBand& bd) {
     Geo g;
     g.L = L;
+    g.H = bd.H;
     const int na = L >> 1;
-    g.nb = L / BY;
+    g.nb = bd.H / BY;
     g.valid = tid < na * g.nb;
     // padding lanes mirror patch (0,0): they load and exchange the same values
     // as thread 0 (identical LDS writes are benign) and never store
@@ -44,7 +59,15 @@ __device__ __forceinline__ Geo make_geo(int L, int tid) {
     g.a = t % na;
     g.b = t / na;
     g.y0 = g.b * BY;
+    const int gy = bd.row0 + g.y0;
+    g.gy0 = gy >= L ? gy - L : gy;
+    g.own = g.valid && g.y0 >= bd.hal && g.y0 < bd.hal + bd.nown;
     return g;
+}
+
+template <int BY>
+__device__ __forceinline__ Geo make_geo(int L, int tid) {
+    return make_geo<BY>(L, tid, RowBand{0, L, 0, L});
 }
 
 // Hopping coefficients of one slice for one patch, pre-scaled by sinh(a_g).
@@ -78,7 +101,7 @@ __device__ __forceinline__ void load_coef(Coef<R, BY>& cf, const R* __restrict__
     const R s1 = NORM ? hp.th1 : hp.sh1, s2 = NORM ? hp.th2 : hp.sh2;
 #pragma unroll
     for (int r = 0; r < BY; ++r) {
-        const int y = g.y0 + r;
+        const int y = g.gy0 + r;
         const cplx<R> p0 = ps[y * L + x0];
         const cplx<R> p1 = ps[y * L + x0 + 1];
         const R pxm = phi_s[2 * (y * L + xm)];
@@ -90,7 +113,7 @@ __device__ __forceinline__ void load_coef(Coef<R, BY>& cf, const R* __restrict__
         cf.ey[0][r + 1] = phase<R>(s2, p0.y);
         cf.ey[1][r + 1] = phase<R>(s2, p1.y);
     }
-    const int ym = g.y0 == 0 ? L - 1 : g.y0 - 1;
+    const int ym = g.gy0 == 0 ? L - 1 : g.gy0 - 1;
     const cplx<R> q0 = ps[ym * L + x0];
     const cplx<R> q1 = ps[ym * L + x0 + 1];
     cf.ey[0][0] = phase<R>(s2, q0.y);
@@ -127,7 +150,7 @@ __device__ __forceinline__ void load_coef_tab(Coef<R, BY>& cf, const cplx<R>* __
     const int xm = x0 == 0 ? L - 1 : x0 - 1;
 #pragma unroll
     for (int r = 0; r < BY; ++r) {
-        const int y = g.y0 + r;
+        const int y = g.gy0 + r;
         cplx<R> h0x, h0y, h1x, h1y;
         ld_pair<R>(hop_s + 2 * (y * L + x0), h0x, h0y);
         ld_pair<R>(hop_s + 2 * (y * L + x0 + 1), h1x, h1y);
@@ -137,7 +160,7 @@ __device__ __forceinline__ void load_coef_tab(Coef<R, BY>& cf, const cplx<R>* __
         cf.ey[0][r + 1] = h0y;
         cf.ey[1][r + 1] = h1y;
     }
-    const int ym = g.y0 == 0 ? L - 1 : g.y0 - 1;
+    const int ym = g.gy0 == 0 ? L - 1 : g.gy0 - 1;
     cf.ey[0][0] = hop_s[2 * (ym * L + x0) + 1];
     cf.ey[1][0] = hop_s[2 * (ym * L + x0 + 1) + 1];
 }
@@ -157,11 +180,11 @@ __device__ __forceinline__ PatchOff patch_off(const Geo& g) {
     constexpr unsigned S = sizeof(cplx<R>);
     const int L = g.L, x0 = 2 * g.a;
     const int xm = x0 == 0 ? L - 1 : x0 - 1;
-    const int ym = g.y0 == 0 ? L - 1 : g.y0 - 1;
+    const int ym = g.gy0 == 0 ? L - 1 : g.gy0 - 1;
     PatchOff o;
-    o.v = (unsigned)(g.y0 * L + x0) * S;
-    o.h = (unsigned)(g.y0 * L + x0) * 2 * S;
-    o.hm = (unsigned)(g.y0 * L + xm) * 2 * S;
+    o.v = (unsigned)(g.gy0 * L + x0) * S;
+    o.h = (unsigned)(g.gy0 * L + x0) * 2 * S;
+    o.hm = (unsigned)(g.gy0 * L + xm) * 2 * S;
     o.hy = (unsigned)(ym * L + x0) * 2 * S + S;
     return o;
 }
@@ -211,9 +234,9 @@ __device__ __forceinline__ void load_coef_buf(Coef<R, BY>& cf, __amdgpu_buffer_r
 // one 16-byte (NV = 2, complex64) LDS access moves both:
 //   x-groups: EL[row][a] (cx=0 site), ER[row][a] (cx=1 site), row = y >> 1
 //   y-groups: BOT[b][a] (r=0 site), TOP[b][a] (r=BY-1 site)
-__host__ __device__ constexpr int xstride(int L) { return (L >> 1) * L; }
+__host__ __device__ constexpr int xstride(int L, int H) { return (L >> 1) * H; }
 template <typename R>
-inline size_t lds_bytes(int L, int NV) { return (size_t)2 * NV * xstride(L) * sizeof(cplx<R>); }
+inline size_t lds_bytes(int L, int NV, int H = 0) { return (size_t)2 * NV * xstride(L, H > 0 ? H : L) * sizeof(cplx<R>); }
 
 // Write / read one slot of NV complex values (vector n's value at slot[n]).
 template <typename R, int NV>
@@ -253,8 +276,8 @@ template <typename R, int BY, int G, int NV>
 __device__ __forceinline__ void exchange(cplx<R> (*v)[2 * BY], cplx<R> (*pr)[BY], cplx<R>* lds, const Geo& g, int& phase) {
     const int L = g.L;
     const int na = L >> 1;
-    const int plane = xstride(L) >> 1;           // na * L/2 slots
-    cplx<R>* buf = lds + (phase & 1) * NV * xstride(L);
+    const int plane = xstride(L, g.H) >> 1;      // na * H/2 slots
+    cplx<R>* buf = lds + (phase & 1) * NV * xstride(L, g.H);
     using IO = SlotIO<R, NV>;
     if constexpr (G == 1 || G == 3) {
         constexpr int rc = (G == 1) ? 1 : 0;   // edge rows of group G have r % 2 == rc
@@ -441,7 +464,7 @@ __device__ __forceinline__ void load_coefc(CoefC<R, BY>& cf, const R* __restrict
     const int x0 = 2 * g.a;
 #pragma unroll
     for (int r = 0; r < BY; ++r) {
-        const int y = g.y0 + r;
+        const int y = g.gy0 + r;
         const cplx<R> p0 = ps[y * L + x0];
         const cplx<R> p1 = ps[y * L + x0 + 1];
         const R t_in = (r & 1) ? hp.th2 : hp.th1;   // g1 (r even) or g3 (r odd)
@@ -609,14 +632,14 @@ __device__ __forceinline__ void apply_Bc(cplx<R> (&v)[2 * BY], const CoefC<R, BY
 // 2 x 16 B (c128) access per row, adjacent threads adjacent in memory.
 template <typename R, int BY>
 __device__ __forceinline__ void load_patch(cplx<R> (&v)[2 * BY], const cplx<R>* __restrict__ s, const Geo& g) {
-    const cplx<R>* p = s + g.y0 * g.L + 2 * g.a;
+    const cplx<R>* p = s + g.gy0 * g.L + 2 * g.a;
 #pragma unroll
     for (int r = 0; r < BY; ++r) ld_pair<R>(p + r * g.L, v[2 * r], v[2 * r + 1]);
 }
 
 template <typename R, int BY>
 __device__ __forceinline__ void store_patch(cplx<R>* __restrict__ s, const cplx<R> (&v)[2 * BY], const Geo& g) {
-    cplx<R>* p = s + g.y0 * g.L + 2 * g.a;
+    cplx<R>* p = s + g.gy0 * g.L + 2 * g.a;
 #pragma unroll
     for (int r = 0; r < BY; ++r) st_pair<R>(p + r * g.L, v[2 * r], v[2 * r + 1]);
 }
