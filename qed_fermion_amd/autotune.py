@@ -3,7 +3,9 @@
 Three shape parameters decide how a launch maps onto the MI355X:
 
 * ``by``  -- patch height of the operator kernels (each thread owns a 2 x by
-  patch of a tau slice; one work-group owns a whole slice), in {2, 4, 6};
+  patch of a tau slice; one work-group owns a whole slice), in {2, 4, 6}; on a
+  lattice too wide for a whole slice of 2 x 2 patches, by = 2 means bands of
+  rows with recomputed halos (several work-groups per slice);
 * ``t``   -- tau slices marched by one operator work-group (the tau pipeline
   pays one extra B application per run, so long runs are cheaper, but fewer
   work-groups fill fewer CUs);
@@ -46,7 +48,9 @@ def _shape_key(prec, L, Lt, nrhs, kind):
 def valid_by(L, prec):
     """Patch heights that fit L (2 x by patches tile the slice; 512-thread
     work-groups, 1024 for by = 2).  complex128 with by >= 4 exceeds the
-    register file, so it is only offered when nothing else fits."""
+    register file, so it is only offered when nothing else fits.  Where a whole
+    slice of 2 x 2 patches does not fit one work-group, by = 2 is offered as
+    bands (the only choice when no other height fits)."""
     out = []
     for by in (2, 4, 6):
         if L % by or by % 2:
@@ -57,6 +61,8 @@ def valid_by(L, prec):
         out.append(by)
     if prec == QF_PREC_F64 and 2 in out:
         out = [2]
+    if 2 not in out and L % 2 == 0 and 2 * (1024 // (L // 2)) >= 18:
+        out.append(2)   # bands of 2 x 2 patches with 8 halo rows (L <= 226)
     return out
 
 

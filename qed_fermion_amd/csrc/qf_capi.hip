@@ -129,6 +129,10 @@ struct BandCfg {
     int nband = 1, H = 0, hal = 0, nown = 0, nthr = 0;
 };
 inline bool square_ok(int L) { return pick_by_default(L, 0) != 0; }
+// A launch with 2x2 patches on a lattice whose whole slice of 2x2 patches exceeds a
+// work-group runs in bands (chosen by the tiling when no patch shape fits, or
+// requested as patch height 2 through the tuning).
+inline bool band_mode(int L, int by) { return by == 2 && !by_ok(L, 2); }
 constexpr int kHalB = 4, kHalMdm = 8;
 
 BandCfg band_cfg(int L, int hal) {
@@ -154,7 +158,7 @@ void set_band(A& a, const BandCfg& c) {
 Tiling tiling(int prec, int nrhs, int L, int Lt) {
     Tiling t;
     const Tuning tu = find_tuning(prec, L, Lt, nrhs);
-    const bool band = !square_ok(L);
+    const bool band = !square_ok(L) || band_mode(L, tu.by);
     t.by = band ? 2 : pick_by(L, prec, tu.by);
     const long long units = (long long)nrhs * Lt;
     // tau slices per operator work-group: about 2048 work-groups in flight
@@ -238,8 +242,7 @@ int launch_mdm_k(const MdmArgs<R>& a0, int nrhs, hipStream_t s) {
     constexpr bool PIPE = mdm_pipe<R, BY>();
     dim3 blk = blk_of(a.L, BY);
     a.nband = 1;
-    if (!square_ok(a.L)) {   // bands of 2x2 patches (BY = 2 from the tiling)
-        if (BY != 2) return fail(QF_ERR_UNSUPPORTED, "L=%d: bands use 2x2 patches", a.L);
+    if (band_mode(a.L, BY)) {   // bands of 2x2 patches
         const BandCfg c = band_cfg(a.L, kHalMdm);
         set_band(a, c);
         blk = dim3((unsigned)(((c.nthr + 63) / 64) * 64));
@@ -298,8 +301,7 @@ int launch_op_by(const OpArgs<R>& a0, int nrhs, hipStream_t s) {
     dim3 blk = blk_of(a.L, BY);
     BandCfg c;
     c.H = a.L;
-    if (!square_ok(a.L)) {
-        if (BY != 2) return fail(QF_ERR_UNSUPPORTED, "L=%d: bands use 2x2 patches", a.L);
+    if (band_mode(a.L, BY)) {
         c = band_cfg(a.L, kHalB);
         blk = dim3((unsigned)(((c.nthr + 63) / 64) * 64));
     }
@@ -337,8 +339,7 @@ int launch_force_by(const ForceArgs<R>& a0, int nrhs, hipStream_t s) {
     dim3 blk = blk_of(a.L, BY);
     BandCfg c;
     c.H = a.L;
-    if (!square_ok(a.L)) {
-        if (BY != 2) return fail(QF_ERR_UNSUPPORTED, "L=%d: bands use 2x2 patches", a.L);
+    if (band_mode(a.L, BY)) {
         c = band_cfg(a.L, kHalMdm);
         blk = dim3((unsigned)(((c.nthr + 63) / 64) * 64));
     }
@@ -821,7 +822,7 @@ int qf_set_tuning(int prec, int by, int t, int tb) {
 
 int qf_set_tuning_shape(int prec, int L, int Ltau, int nrhs, int by, int t, int tb) {
     if (prec != QF_PREC_F32 && prec != QF_PREC_F64) return fail(QF_ERR_INVALID, "prec %d", prec);
-    if (by != 0 && !by_ok(L, by) && !(by == 2 && !square_ok(L))) return fail(QF_ERR_INVALID, "patch height %d does not fit L=%d", by, L);
+    if (by != 0 && !by_ok(L, by) && !(by == 2 && band_cfg(L, kHalMdm).nthr > 0)) return fail(QF_ERR_INVALID, "patch height %d does not fit L=%d", by, L);
     if (t < 0 || (tb != 0 && !tb_ok(tb)) || nrhs < 0 || L < 2 || Ltau < 1) return fail(QF_ERR_INVALID, "bad shape or tile size");
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (auto& e : g_shape_tune)
@@ -1160,19 +1161,27 @@ int qf_ragged_plan(int nprob, const int* Ls, const int* Lts, void* plan, size_t 
     for (int k = 0; k < nprob; ++k) {
         const int L = Ls[k], Lt = Lts[k];
         if (L < 2 || (L & 1)) return fail(QF_ERR_ODD_L, "lattice %d: L=%d must be even and >= 2", k, L);
-        if (!by_ok(L, 2)) return fail(QF_ERR_UNSUPPORTED, "lattice %d: L=%d exceeds the 1024-thread work-group of 2x2 patches", k, L);
         if (Lt < 1) return fail(QF_ERR_INVALID, "lattice %d: Ltau=%d", k, Lt);
         Prob& q = P[k];
         q.L = L; q.Lt = Lt;
         q.T = std::min(Lt, 4);
         q.nA = (Lt + q.T - 1) / q.T;
+        q.nband = 1; q.H = L; q.hal = 0; q.nown = L;
+        int thr = nthreads(L, 2);
+        if (!by_ok(L, 2)) {   // too wide for one work-group per slice: bands of 2x2 patches
+            const BandCfg c = band_cfg(L, kHalMdm);
+            if (c.nthr == 0) return fail(QF_ERR_UNSUPPORTED, "lattice %d: L=%d has no band of 2x2 patches that fits (L <= 226)", k, L);
+            q.nband = c.nband; q.H = c.H; q.hal = c.hal; q.nown = c.nown;
+            q.nA *= c.nband;
+            thr = c.nthr;
+        }
         q.wg0 = (int)wg;
         q.nsx = (L * L + cgu_tx(h->tb) - 1) / cgu_tx(h->tb);
         q.nB = q.nsx * ((Lt + h->tb - 1) / h->tb);
         q.bwg0 = (int)bwg;
         q.voff = voff; q.poff = 2 * voff; q.hoff = 2 * voff; q.toff = toff;
         q.pa0 = (int)pa; q.pb0 = (int)pb;
-        maxthr = std::max(maxthr, nthreads(L, 2));
+        maxthr = std::max(maxthr, thr);
         wg += q.nA; bwg += q.nB; pa += q.nA; pb += q.nB;
         voff += (long long)L * L * Lt; toff += 13LL * Lt;
         if (wg > (1LL << 30) || bwg > (1LL << 30)) return fail(QF_ERR_INVALID, "ragged batch too large");

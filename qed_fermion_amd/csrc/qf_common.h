@@ -125,6 +125,7 @@ struct Prob {
     long long hoff;   // hop table (complex)
     long long toff;   // preconditioner table (13 * Lt reals)
     int pa0, pb0;     // operator / update partial sums
+    int nband, H, hal, nown;   // bands of a lattice too wide for one work-group per slice (nband 1: none)
 };
 struct RaggedHead {
     int nprob, by, nthreads;    // patch height and work-group size of the operator launch

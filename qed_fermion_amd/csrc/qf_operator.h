@@ -144,8 +144,9 @@ __device__ __forceinline__ void mdagm_body(const MdmArgs<R>& A, int bid, int nbl
         for (int j = 1; j < A.nprob; ++j)
             if (A.prob[j].wg0 <= bid) k = j;
         const Prob& P = A.prob[k];
-        rho = k; run = bid - P.wg0; rest = run; L = P.L; Lt = P.Lt; T = P.T; nBk = P.nB;
-        bd = RowBand{0, L, 0, L};
+        const int nbd = P.nband > 1 ? P.nband : 1;
+        rho = k; rest = bid - P.wg0; run = rest / nbd; L = P.L; Lt = P.Lt; T = P.T; nBk = P.nB;
+        bd = band_of(L, P.nband, P.H, P.hal, P.nown, rest - run * nbd);
         vo = P.voff; pho = P.poff; hoo = P.hoff; pa = P.pa0; pb = P.pb0;
     } else {
         // per rhs the work-groups are (run, band), band fastest
@@ -376,7 +377,7 @@ __device__ __forceinline__ void mdagm_body(const MdmArgs<R>& A, int bid, int nbl
 #pragma unroll
             for (int k = 0; k < 2 * BY; ++k) Yn[k] = W[k];
             if (more) {   // x_{tau+1} again
-                if (MODE == MDM_CG && A.nband > 1) {
+                if (MODE == MDM_CG && bd.hal > 0) {
                     // a band's halo rows of p are stored by the neighbouring bands: recompute them
                     Raw rr;
                     ldraw(rr, tn);

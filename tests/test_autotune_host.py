@@ -18,3 +18,11 @@ def test_refine_scales_per_iteration_times(monkeypatch):
     table = {"x": 5.0, "y": 0.5}
     autotune._refine(table, lambda k: None, lambda: None, reps=1, scale=1.0 / 20, top=2, rounds=1)
     assert table == {"x": 1.0, "y": 0.5}
+
+
+def test_valid_patch_heights_include_bands_where_a_slice_does_not_fit():
+    f32, f64 = 0, 1
+    assert autotune.valid_by(36, f32) == [2, 4, 6] and autotune.valid_by(36, f64) == [2]
+    assert autotune.valid_by(66, f32) == [6, 2] and autotune.valid_by(66, f64) == [6, 2]   # 33 x 33 patches: bands
+    assert autotune.valid_by(70, f32) == [2] and autotune.valid_by(96, f64) == [2]        # bands only
+    assert autotune.valid_by(226, f32) == [2] and autotune.valid_by(228, f32) == []
