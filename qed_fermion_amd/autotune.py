@@ -66,8 +66,22 @@ def valid_by(L, prec):
     return out
 
 
-def _t_candidates(Lt):
-    return sorted({t for t in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 14, 15, 16, 19, 22, 24) if t <= Lt})
+def _t_candidates(Lt, nrhs=None, ncu=None):
+    """tau-run lengths to time: a fixed ladder, plus for nrhs right-hand sides the shortest runs
+    whose work-groups (nrhs * ceil(Lt / t)) still fit k per CU for k = 1..4, so that the last
+    round of work-groups is not a partial one (32 chains at Ltau = 360: t = 23 fills 512
+    work-groups, where t = 24 leaves 480)."""
+    out = {t for t in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 14, 15, 16, 19, 22, 24) if t <= Lt}
+    if nrhs:
+        if ncu is None:
+            ncu = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
+        for k in (1, 2, 3, 4):
+            runs = (k * ncu) // nrhs
+            if runs >= 1:
+                t = -(-Lt // runs)
+                if t <= min(Lt, 48):
+                    out.add(t)
+    return sorted(out)
 
 
 def _time(fn, reps, warm=2):
@@ -119,7 +133,7 @@ def tune_operator(prec, L, Lt, nrhs, shared_field=True, reps=10, seed=0, source=
         raise ValueError(f"source {source!r}")
     table = {}
     for by in valid_by(L, prec):
-        for t in _t_candidates(Lt):
+        for t in _t_candidates(Lt, nrhs):
             kernels.set_tuning_shape(prec, L, Lt, nrhs, by, t, 0)
             table[(by, t)] = _time(run, reps)
     _refine(table, lambda k: kernels.set_tuning_shape(prec, L, Lt, nrhs, k[0], k[1], 0), run, reps)
@@ -145,7 +159,7 @@ def tune_pcg(prec, L, Lt, nrhs, iters=20, reps=3, seed=0, tb_list=(16, 32, 64, 1
 
     table = {}
     for by in valid_by(L, prec):
-        for t in _t_candidates(Lt):
+        for t in _t_candidates(Lt, nrhs):
             kernels.set_tuning_shape(prec, L, Lt, nrhs, by, t, 0)
             table[(by, t, 0)] = _time(run, reps, warm=1) / iters
     (by, t, _), _ = min(table.items(), key=lambda kv: kv[1])

@@ -26,3 +26,11 @@ def test_valid_patch_heights_include_bands_where_a_slice_does_not_fit():
     assert autotune.valid_by(66, f32) == [6, 2] and autotune.valid_by(66, f64) == [6, 2]   # 33 x 33 patches: bands
     assert autotune.valid_by(70, f32) == [2] and autotune.valid_by(96, f64) == [2]        # bands only
     assert autotune.valid_by(226, f32) == [2] and autotune.valid_by(228, f32) == []
+
+
+def test_tau_run_candidates_fill_whole_rounds_of_work_groups():
+    c = autotune._t_candidates(360, nrhs=32, ncu=256)
+    assert 23 in c and 24 in c              # 32 x 16 runs = 512 work-groups: two per CU
+    assert 45 in c                          # one per CU
+    assert max(c) <= 48 and all(t <= 360 for t in c)
+    assert autotune._t_candidates(10) == list(range(1, 11))

@@ -145,3 +145,30 @@ def test_fp64_residual_1024_thread_work_group(dev):
         k.set_tuning(1, 0, 0, 0)
     res = relerr(O.apply_MdagM(phi[0], xm[0], L, Lt), b[0])
     assert res <= 1.5e-10 and errm[0].item() <= 1e-10 and itm[0].item() < 300, (res, errm[0].item())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_band_trajectory(dev, precision):
+    """A whole HMC trajectory on a banded lattice (L=70, Ltau=30, 3 leapfrog steps, every solve
+    to 1e-10) against the oracle's trajectory: final field and momenta within 1e-8 (FP64) or 1e-7
+    (mixed), H0 to 1e-9 and Delta H to 1e-6."""
+    from qed_fermion_amd.trajectory import TrajectoryEngine
+    L, Lt, nl = 70, 30, 3
+    print(f"[band trajectory] {precision}", flush=True)
+    phi = make_phi(L, Lt, "gauss", seed=7001)
+    p = np.random.default_rng(7002).standard_normal(phi.shape)
+    R = make_vec(L, Lt, seed=7003)
+    eng = TrajectoryEngine(L, Lt, 1, precision=precision, max_iter=1500, rtol=1e-10, n_leap=nl, delta_t=0.05, device="cuda")
+    eng.set_phi(torch.from_numpy(phi.reshape(1, -1)).cuda())
+    eng.p.copy_(torch.from_numpy(p.reshape(1, -1)))
+    eng.R.copy_(torch.from_numpy(R))
+    eng.propose(draw=False)
+    torch.cuda.synchronize()
+    x1, p1, H0, H1, err = O.trajectory(phi[0], p[0], R[0], L, Lt, dt=0.05, n_leap=nl, table=O.precon_table(Lt), rtol=1e-10,
+                                       max_iter=1500)
+    tol = 1e-8 if precision == "fp64" else 1e-7
+    assert eng.err.max().item() <= 1e-10 and err <= 1e-10
+    assert relerr(eng.x[0].cpu().numpy(), x1) < tol
+    assert relerr(eng.p[0].cpu().numpy(), p1) < tol
+    assert abs(eng.H0[0].item() - H0) < 1e-9 * abs(H0)
+    assert abs((eng.H1[0] - eng.H0[0]).item() - (H1 - H0)) < 1e-6
