@@ -362,8 +362,10 @@ int launch_force_by(const ForceArgs<R>& a0, int nrhs, hipStream_t s) {
 }
 
 template <typename R, int MODE, bool PRE>
-int launch_cgu_p(const CguArgs<R>& a, const Tiling& t, int nrhs, hipStream_t s) {
-    const dim3 grid(t.nsx, (a.Lt + t.TB - 1) / t.TB, nrhs), blk(512);
+int launch_cgu_p(const CguArgs<R>& a0, const Tiling& t, int nrhs, hipStream_t s) {
+    const dim3 grid(t.nsx, (a0.Lt + t.TB - 1) / t.TB, nrhs), blk(512);
+    CguArgs<R> a = a0;
+    a.xcd = ((long long)a.Vs * (long long)sizeof(cplx<R>)) % 128 != 0;   // slices that end inside a line
     switch (t.TB) {
         case 16: hipLaunchKernelGGL((cg_update_kernel<R, MODE, PRE, 32, 16, 1>), grid, blk, 0, s, a); break;
         case 32: hipLaunchKernelGGL((cg_update_kernel<R, MODE, PRE, 16, 32, 1>), grid, blk, 0, s, a); break;

@@ -59,6 +59,7 @@ struct CguArgs {
     int* brk;              // [nrhs] breakdown flag of the last ITER step (INIT clears it; optional)
     const Prob* prob;      // ragged batch (nullptr: uniform; grid (nsx, tiles, nrhs))
     int nprob;
+    int xcd;               // uniform grid: 1 runs the tiles in XCD order (cg_update_kernel)
 };
 
 // A block of TX * TT / SPT threads owns a tile of TX sites x TT slices; each
@@ -295,10 +296,19 @@ __device__ __forceinline__ void cgu_body(const CguArgs<R>& A, int lin, int bx_, 
     }
 }
 
+// A tile's rows of 16 sites are whole 128-byte lines only where a slice is a whole number of
+// lines; elsewhere (complex64 at L = 66: 4356 sites, 34848 B) each row shares a line with the
+// next column's row, and with A.xcd the tiles run in XCD order (xcd_order) so that one XCD runs
+// both columns and fetches the line once (the launcher sets it for such slices only: on aligned
+// slices the order measured a few percent slower).
 template <typename R, int MODE, bool PRE, int TX, int TT, int SPT>
 __global__ void __launch_bounds__(TX * TT / SPT) cg_update_kernel(CguArgs<R> A) {
-    cgu_body<R, MODE, PRE, TX, TT, SPT>(A, (int)blockIdx.x, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.x,
-                                        (int)gridDim.y);
+    const int gx = (int)gridDim.x, gy = (int)gridDim.y;
+    const int G = gx * gy * (int)gridDim.z;
+    const int w = (int)blockIdx.x + gx * ((int)blockIdx.y + gy * (int)blockIdx.z);
+    const int l = A.xcd ? xcd_order(w, G) : w;
+    const int bx = l % gx, rest = l / gx;
+    cgu_body<R, MODE, PRE, TX, TT, SPT>(A, l, bx, rest % gy, rest / gy, gx, gy);
 }
 
 // Scalars after the last iteration (the reference returns the error of the

@@ -51,6 +51,17 @@ template <> __device__ __forceinline__ void qsincos<double>(double a, double* s,
 // across the exchange barriers of the current one.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
+// The MI355X places the work-groups of a launch on its 8 XCDs round-robin in dispatch order
+// (w = 0, 8, 16, ... on the first).  xcd_order(w, G) renumbers them so that each XCD takes one
+// contiguous range of logical work items, in order: neighbouring items then run on one XCD at
+// about the same time and share its L2 (a bijection of [0, G) for any G).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_order(int w, int G) {
+    const int q = G / kXcds, r = G - q * kXcds;
+    const int j = w % kXcds, k = w / kXcds;
+    return j * q + min(j, r) + k;
+}
+
 // v + h*w (normalised pair update: 4 FMAs)
 template <typename R> __device__ __forceinline__ cplx<R> pair_add(cplx<R> v, cplx<R> h, cplx<R> w) {
     return mk<R>(fma(-h.y, w.y, fma(h.x, w.x, v.x)), fma(h.y, w.x, fma(h.x, w.y, v.y)));
