@@ -3,7 +3,9 @@ HMC (leapfrog_proposer5 + metropolis_update restated in oracle/qf_oracle.c, with
 reference's draws: p ~ N(0, m), R = randn(complex)/sqrt(2)) on L=4, Ltau=30, for comparison
 with the GPU sampler run under the same protocol (tests/test_gpu_statistics.py).  Per chain:
 the means over the measured trajectories of S_tau and S_plaq of the current field and of the
-acceptance.  64 chains from phi = 0, 8 worker processes with one OpenMP thread each; about two
+acceptance, and over every fermion_every-th measured trajectory of the fermion observable B_x:
+the mean over sites of the reference's x-bond field B_r (stochastic_estimator.py:1631-1654) in
+its exact noise average from M^{-1} (tests/dense_ref.py bond_x).  64 chains from phi = 0, 8 worker processes with one OpenMP thread each; about two
 minutes on 8 host cores:
 
     python tests/golden/make_hmc_statistics.py
@@ -18,14 +20,16 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 
 PROTOCOL = {"L": 4, "Ltau": 30, "dtau": 0.1, "J": 1.0, "K": 1.0, "compact": False, "m": 1.0, "dt": 0.1, "n_leap": 5,
-            "rtol": 1e-10, "max_iter": 400, "start": "phi = 0", "n_therm": 100, "n_meas": 300}
+            "rtol": 1e-10, "max_iter": 400, "start": "phi = 0", "n_therm": 100, "n_meas": 300, "fermion_every": 5}
 CHAINS, SEED = 64, 2024
 
 
 def run_chain(c):
     os.environ["OMP_NUM_THREADS"] = "1"
     sys.path.insert(0, ROOT)
+    sys.path.insert(0, os.path.dirname(HERE))
     import numpy as np
+    import dense_ref as D
     from oracle import oracle as O
     P = PROTOCOL
     L, Lt = P["L"], P["Ltau"]
@@ -34,7 +38,8 @@ def run_chain(c):
     tab = O.precon_table(Lt)
     rng = np.random.default_rng([SEED, c])
     phi = np.zeros((Lt, L, L, 2))
-    sums = np.zeros(3)
+    sums = np.zeros(4)
+    nf = 0
     for k in range(P["n_therm"] + P["n_meas"]):
         p = rng.standard_normal(phi.shape) * math.sqrt(P["m"])
         R = (rng.standard_normal(N) + 1j * rng.standard_normal(N)) / 2.0
@@ -45,14 +50,17 @@ def run_chain(c):
         if acc:
             phi = x1
         if k >= P["n_therm"]:
-            sums += (O.action_tau(phi, L, Lt, cJ, P["compact"]), O.action_plaq(phi, L, Lt, cK), float(acc))
-    return (sums / P["n_meas"]).tolist()
+            sums[:3] += (O.action_tau(phi, L, Lt, cJ, P["compact"]), O.action_plaq(phi, L, Lt, cK), float(acc))
+            if (k - P["n_therm"]) % P["fermion_every"] == 0:
+                sums[3] += D.bond_x(L, Lt, phi, dtau=P["dtau"]).real.mean()
+                nf += 1
+    return (sums[:3] / P["n_meas"]).tolist() + [sums[3] / nf]
 
 
 def main():
     with mp.get_context("spawn").Pool(8) as pool:
         chains = pool.map(run_chain, range(CHAINS))
-    names = ("S_tau", "S_plaq", "acceptance")
+    names = ("S_tau", "S_plaq", "acceptance", "B_x")
     out = {"protocol": PROTOCOL, "chains": CHAINS, "seed": SEED, "chain_means": {n: [c[i] for c in chains] for i, n in enumerate(names)}}
     for i, n in enumerate(names):
         v = [c[i] for c in chains]

@@ -4,8 +4,11 @@ on-device draws of p and R and its on-device Metropolis step) run under the prot
 tests/golden/make_hmc_statistics.py, whose oracle chains give the expected means.  The
 deterministic parity tests fix the trajectory for given (phi, p, R); this checks what they
 leave open: the distributions of the draws and the accept/reject step, through the run
-averages of S_tau, S_plaq (of the current field, evaluated by the oracle on both sides) and
-the acceptance, each within 4 combined standard errors over independent chains."""
+averages of S_tau, S_plaq (of the current field, evaluated by the oracle on both sides), the
+acceptance and a fermion observable, B_x: the mean x-bond of the reference's Tv/B_r
+(stochastic_estimator.py:1631-1654) in its exact noise average from M^{-1} of the current field
+(tests/dense_ref.py bond_x, evaluated the same way on both sides every fermion_every-th measured
+trajectory), each within 4 combined standard errors over independent chains."""
 import json
 import math
 import os
@@ -14,6 +17,7 @@ import numpy as np
 import pytest
 import torch
 
+import dense_ref as D
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -33,7 +37,8 @@ def test_sampler_statistics_match_the_oracle(dev):
     eng.capture(n_warmup=1)
     eng.set_phi(torch.zeros(bs, 2 * L * L * Lt, dtype=torch.float64, device=dev))
     cJ, cK = O.couplings(P["J"], P["K"], P["dtau"])
-    sums = np.zeros((bs, 3))
+    sums = np.zeros((bs, 4))
+    nf = 0
     worst = 0.0
     for k in range(P["n_therm"] + P["n_meas"]):
         eng.replay()
@@ -41,12 +46,16 @@ def test_sampler_statistics_match_the_oracle(dev):
             x = eng.x.cpu().numpy().reshape(bs, Lt, L, L, 2)
             acc = eng.accept.cpu().numpy()
             worst = max(worst, eng.err.max().item())
+            ferm = (k - P["n_therm"]) % P["fermion_every"] == 0
+            nf += ferm
             for c in range(bs):
-                sums[c] += (O.action_tau(x[c], L, Lt, cJ, P["compact"]), O.action_plaq(x[c], L, Lt, cK), float(acc[c]))
+                sums[c, :3] += (O.action_tau(x[c], L, Lt, cJ, P["compact"]), O.action_plaq(x[c], L, Lt, cK), float(acc[c]))
+                if ferm:
+                    sums[c, 3] += D.bond_x(L, Lt, x[c], dtau=P["dtau"]).real.mean()
     torch.cuda.synchronize()
     assert (eng.status == 0).all() and worst <= P["rtol"], worst    # every solve converged within its 150 iterations
-    means = sums / P["n_meas"]
-    for i, name in enumerate(("S_tau", "S_plaq", "acceptance")):
+    means = np.concatenate([sums[:, :3] / P["n_meas"], sums[:, 3:] / nf], axis=1)
+    for i, name in enumerate(("S_tau", "S_plaq", "acceptance", "B_x")):
         m = means[:, i].mean()
         se = means[:, i].std(ddof=1) / math.sqrt(bs)
         r = ref[name]

@@ -165,3 +165,30 @@ def test_bond_bond_combination_and_k_grid():
     assert np.allclose(out["BB_r"], (2 * t + 4 * tvv).real)
     # BB_k at the grid centre (index L/2, L/2) is the k = 0 mode: sum over Delta of BB_r
     assert np.isclose(out["BB_k"][L // 2, L // 2], (2 * t + 4 * tvv).sum().real)
+
+
+def test_tv_noise_average_is_the_exact_bond():
+    """Tv over the exact noise basis (every unit vector with each of the phases 1, i, -1, -i:
+    the sample mean of eta*_i eta_k is delta_ik / N) equals the dense M^{-1} bond of
+    dense_ref.bond_x, the fermion observable of the sampler's statistics test; its diagonal
+    blocks equal those of the dense inverse; a transposed M^{-1} (the plausible slip) differs,
+    since M^{-1} is not Hermitian."""
+    L, Lt = 4, 4
+    N = L * L * Lt
+    phi = make_phi(L, Lt, "uniform", seed=12)[0]
+    Minv = np.linalg.inv(D.M_mat(L, Lt, phi))
+    phases = np.array([1, 1j, -1, -1j])
+    eta = (phases[:, None, None] * np.eye(N)[None]).reshape(4 * N, N)
+    G = (Minv @ eta.T).T
+    _, tv = OB.Tvv(eta, G, _boson(phi), L, Lt)
+    want = D.bond_x(L, Lt, phi)
+    assert relerr(tv * N, want) < 1e-10
+    G = D.equal_time_G(L, Lt, phi)
+    for tau in range(Lt):
+        blk = Minv[tau * L * L:(tau + 1) * L * L, tau * L * L:(tau + 1) * L * L]
+        assert relerr(G[tau], blk) < 1e-12
+    tt, yy, xx = np.meshgrid(np.arange(Lt), np.arange(L), np.arange(L), indexing="ij")
+    r, rx = tt * L * L + yy * L + xx, tt * L * L + yy * L + (xx + 1) % L
+    u = np.exp(1j * phi[..., 0])
+    slip = (Minv[r, rx] * u + Minv[rx, r] * np.conj(u)).mean(axis=0)
+    assert relerr(slip, want) > 1e-2

@@ -206,7 +206,7 @@ def test_force_against_autograd():
     L, Lt = 4, 4
     phi = make_phi(L, Lt, "uniform", seed=23)[0]
     psi = make_vec(L, Lt, seed=24)[0]
-    lists = D.bond_lists(L)
+    lists = [(i.copy(), j.copy(), mu) for i, j, mu in D.bond_lists(L)]
     ph = torch.tensor(phi, dtype=torch.float64, requires_grad=True)
     Vs = L * L
     Ms = torch.eye(Vs * Lt, dtype=torch.complex128)
