@@ -96,7 +96,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--tiling", default=None,
                     help="with --probe pcg: patch height, tau run and update tile as BY,T,TB instead of the autotuner; "
-                         "with the operator probes: one or more such tilings separated by ';', three launches each")
+                         "with the operator probes: one or more such tilings separated by ';', three launches each, "
+                         "each optionally with a fourth field NS (right-hand sides per work-group)")
     return ap.parse_args()
 
 
@@ -229,7 +230,7 @@ def traffic_of(L, Lt, nrhs, kind, tiling):
         return None
     for e in rec.get("entries", []):
         if (e.get("L"), e.get("Ltau"), e.get("nrhs"), e.get("kind")) == (L, Lt, nrhs, kind) and \
-                (e["tiling"]["by"], e["tiling"]["t"]) == (tiling["by"], tiling["t"]):
+                (e["tiling"]["by"], e["tiling"]["t"], e["tiling"].get("ns", 1)) == (tiling["by"], tiling["t"], tiling.get("ns", 1)):
             return e.get("traffic_bytes")
     return None
 
@@ -709,7 +710,8 @@ def probe(args):
     order = []
     for tl in tilings:
         if tl is not None:
-            K.set_tuning_shape(0, L, Lt, n, *tl)
+            K.set_tuning_shape(0, L, Lt, n, *tl[:3])
+            K.set_tuning_ns(0, L, Lt, n, tl[3] if len(tl) > 3 else 0)
         for _ in range(3):
             run()
         torch.cuda.synchronize()

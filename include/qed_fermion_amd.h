@@ -79,6 +79,16 @@ int qf_clear_tuning(void);
 /* out4 = {patch height, tau slices per operator work-group, tau slices per
  * update work-group, operator work-groups per right-hand side}. */
 int qf_get_tiling(int prec, int nrhs, int L, int Ltau, int* out4);
+/* Right-hand sides per work-group of the plain M^dag M from phi (qf_apply
+ * QF_OP_MDAGM) for one lattice shape (nrhs = 0: any batch size): ns > 1 runs
+ * the same tau run of ns consecutive right-hand sides side by side in one
+ * work-group, to fill whole wavefronts where one slice leaves a wavefront mostly
+ * empty (L=36: 324 lanes per slice, 972 for three); 0 or 1 = one.  Whole-slice
+ * tilings only; capped by the 1024-thread work-group (512 above 2x2 patches).
+ * Set by the autotuner; no counterpart in the reference. */
+int qf_set_tuning_ns(int prec, int L, int Ltau, int nrhs, int ns);
+/* The right-hand sides per work-group qf_apply's M^dag M from phi would use. */
+int qf_get_rhs_per_group(int prec, int nrhs, int L, int Ltau, int* ns);
 /* PCG x updates in pairs (on = 1, the default): an iteration that is not
  * followed by a read of x holds x += alpha p and the next iteration applies
  * both steps, so x and p are read every other iteration.  on = 0 updates x in

@@ -65,6 +65,8 @@ SIGNATURES = {
     "qf_clear_tuning": (_I, []),
     "qf_set_paired_x_updates": (_I, [_I]),
     "qf_get_tiling": (_I, [_I, _I, _I, _I, _P]),
+    "qf_set_tuning_ns": (_I, [_I, _I, _I, _I, _I]),
+    "qf_get_rhs_per_group": (_I, [_I, _I, _I, _I, _P]),
     "qf_apply": (_I, [_I, _I, _P, _LL, _P, _P, _I, _I, _I, _D, _D, _P]),
     "qf_hop_table": (_I, [_I, _P, _LL, _I, _I, _I, _D, _D, _P, _P]),
     "qf_apply_hop": (_I, [_I, _I, _P, _LL, _P, _P, _I, _I, _I, _D, _D, _P]),

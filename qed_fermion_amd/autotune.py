@@ -10,7 +10,10 @@ Three shape parameters decide how a launch maps onto the MI355X:
   pays one extra B application per run, so long runs are cheaper, but fewer
   work-groups fill fewer CUs);
 * ``tb``  -- tau slices of a PCG update tile (16, 32, 64 or 128; the tile's
-  13-slice preconditioner window re-reads 12 halo slices).
+  13-slice preconditioner window re-reads 12 halo slices);
+* ``ns``  -- right-hand sides per work-group of the plain M^dag M from phi: several
+  slices side by side fill whole wavefronts where one slice does not (L=36: 324 lanes
+  in six wavefronts, 648 in eleven).
 
 The reference picks a fixed CUDA block (BLOCK_SIZE = (4, 8),
 hmc_sampler_batch.py:111); here the best values depend on L, Ltau, the batch
@@ -66,11 +69,21 @@ def valid_by(L, prec):
     return out
 
 
+def ns_candidates(L, by, nrhs):
+    """Right-hand sides per work-group to time for the plain M^dag M from phi: 1 up to what
+    one work-group holds (1024 threads with 2 x 2 patches, 512 above) and the batch."""
+    if by <= 0 or L % by or nrhs <= 1:
+        return [1]
+    nts = (L // 2) * (L // by)
+    top = min(nrhs, (1024 if by == 2 else 512) // nts, 4)
+    return list(range(1, max(top, 1) + 1))
+
+
 def _t_candidates(Lt, nrhs=None, ncu=None):
-    """tau-run lengths to time: a fixed ladder, plus for nrhs right-hand sides the shortest runs
-    whose work-groups (nrhs * ceil(Lt / t)) still fit k per CU for k = 1..4, so that the last
-    round of work-groups is not a partial one (32 chains at Ltau = 360: t = 23 fills 512
-    work-groups, where t = 24 leaves 480)."""
+    """tau-run lengths to time: a fixed ladder, plus for nrhs right-hand sides (work-group
+    columns) the shortest runs whose work-groups (nrhs * ceil(Lt / t)) still fit k per CU for
+    k = 1..4, so that the last round of work-groups is not a partial one (32 chains at
+    Ltau = 360: t = 23 fills 512 work-groups, where t = 24 leaves 480)."""
     out = {t for t in (1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 14, 15, 16, 19, 22, 24) if t <= Lt}
     if nrhs:
         if ncu is None:
@@ -113,10 +126,10 @@ def _dtypes(prec):
 
 
 def tune_operator(prec, L, Lt, nrhs, shared_field=True, reps=10, seed=0, source="hop"):
-    """Best (by, t) for M^dag M on nrhs right-hand sides, applied from the hop table
-    (source "hop") or from phi (source "phi", the coefficients' sincos in the kernel);
-    one field shared by all RHS if shared_field, else one per RHS.  Returns
-    (by, t, ms_per_launch, table)."""
+    """Best (by, t, ns) for M^dag M on nrhs right-hand sides, applied from the hop table
+    (source "hop") or from phi (source "phi", the coefficients' sincos in the kernel; ns
+    right-hand sides per work-group are timed only there); one field shared by all RHS if
+    shared_field, else one per RHS.  Returns (by, t, ms_per_launch, table, ns)."""
     rd, cd = _dtypes(prec)
     g = torch.Generator(device="cpu").manual_seed(seed)
     N = L * L * Lt
@@ -131,15 +144,20 @@ def tune_operator(prec, L, Lt, nrhs, shared_field=True, reps=10, seed=0, source=
         run = lambda: kernels.apply("MdagM", phi, x, L, Lt, out=y)
     else:
         raise ValueError(f"source {source!r}")
+    def select(k):
+        kernels.set_tuning_shape(prec, L, Lt, nrhs, k[0], k[1], 0)
+        kernels.set_tuning_ns(prec, L, Lt, nrhs, k[2])
+
     table = {}
     for by in valid_by(L, prec):
-        for t in _t_candidates(Lt, nrhs):
-            kernels.set_tuning_shape(prec, L, Lt, nrhs, by, t, 0)
-            table[(by, t)] = _time(run, reps)
-    _refine(table, lambda k: kernels.set_tuning_shape(prec, L, Lt, nrhs, k[0], k[1], 0), run, reps)
-    kernels.set_tuning_shape(prec, L, Lt, nrhs, 0, 0, 0)
-    (by, t), ms = min(table.items(), key=lambda kv: kv[1])
-    return by, t, ms, table
+        for ns in (ns_candidates(L, by, nrhs) if source == "phi" else [1]):
+            for t in _t_candidates(Lt, -(-nrhs // ns)):
+                select((by, t, ns))
+                table[(by, t, ns)] = _time(run, reps)
+    _refine(table, select, run, reps)
+    select((0, 0, 0))
+    (by, t, ns), ms = min(table.items(), key=lambda kv: kv[1])
+    return by, t, ms, table, ns
 
 
 def tune_pcg(prec, L, Lt, nrhs, iters=20, reps=3, seed=0, tb_list=(16, 32, 64, 128)):
@@ -209,16 +227,19 @@ def tune(prec, L, Lt, nrhs, kind="pcg", persist=True, force=False, **kw):
         if kind == "pcg":
             by, t, tb, ms, _ = tune_pcg(prec, L, Lt, nrhs, **kw)
         elif kind == "operator":
-            by, t, ms, _ = tune_operator(prec, L, Lt, nrhs, **kw)
+            by, t, ms, _, ns = tune_operator(prec, L, Lt, nrhs, **kw)
             tb = 0
         else:
             raise ValueError(f"kind {kind!r}")
         hit = {"by": by, "t": t, "tb": tb, "ms": ms}
+        if kind == "operator":
+            hit["ns"] = ns
         if persist:
             cache = _load_cache(path)
             cache.setdefault(dk, {})[key] = hit
             _save_cache(path, cache)
     kernels.set_tuning_shape(prec, L, Lt, nrhs, hit["by"], hit["t"], hit["tb"])
+    kernels.set_tuning_ns(prec, L, Lt, nrhs, hit.get("ns", 0))
     return dict(hit)
 
 
@@ -228,7 +249,8 @@ def load(path=None):
     cache = _load_cache(path or cache_path()).get(device_key(), {})
     n = 0
     for key, v in cache.items():
-        _kind, prec, L, Lt, nrhs = key.split(":")
+        _kind, prec, L, Lt, nrhs = key.split(":")[:5]
         kernels.set_tuning_shape(int(prec), int(L), int(Lt), int(nrhs), v["by"], v["t"], v["tb"])
+        kernels.set_tuning_ns(int(prec), int(L), int(Lt), int(nrhs), v.get("ns", 0))
         n += 1
     return n

@@ -59,6 +59,7 @@ int fail(int code, const char* fmt, ...) {
 // ---------------------------------------------------------------- tiling
 struct Tuning {
     int by = 0, t = 0, tb = 0;   // 0 = automatic
+    int ns = 0;                  // right-hand sides per work-group of the plain M^dag M from phi (0: one)
 };
 Tuning g_tune[2];                // per precision: applies to every shape
 // Per-shape entries (set by the autotuner); nrhs = 0 matches any batch.
@@ -80,6 +81,7 @@ inline bool by_ok(int L, int by) { return by > 0 && L % by == 0 && by % 2 == 0 &
 struct Tiling {
     int by, T, nA;     // operator kernels
     int TB, nsx, nB;   // update kernel: tile of TB slices x cgu_tx(TB) sites
+    int ns;            // right-hand sides per work-group of the plain M^dag M from phi
 };
 
 // Update-kernel tiles: 512 threads over TB in {16, 32, 64, 128} tau slices of
@@ -155,6 +157,18 @@ void set_band(A& a, const BandCfg& c) {
     a.nband = c.nband; a.H = c.H; a.hal = c.hal; a.nown = c.nown;
 }
 
+// Right-hand sides per work-group of the plain M^dag M from phi (whole slices): up to `want`,
+// as many slices as fit one work-group (1024 threads with 2 x 2 patches, else 512), and the
+// group's vectors within the 32-bit offsets of its loads.
+int rhs_per_group(int L, int Lt, int by, int prec, int nrhs, int want) {
+    if (want <= 1 || nrhs <= 1 || by <= 0 || !by_ok(L, by)) return 1;
+    const int nts = nthreads(L, by);
+    const long long vbytes = (long long)L * L * Lt * (prec == QF_PREC_F64 ? 16 : 8);
+    int ns = std::min(std::min(want, nrhs), (by == 2 ? 1024 : 512) / nts);
+    while (ns > 1 && ns * vbytes >= (1LL << 31)) --ns;
+    return std::max(ns, 1);
+}
+
 Tiling tiling(int prec, int nrhs, int L, int Lt) {
     Tiling t;
     const Tuning tu = find_tuning(prec, L, Lt, nrhs);
@@ -174,6 +188,7 @@ Tiling tiling(int prec, int nrhs, int L, int Lt) {
     t.TB = tu.tb ? tu.tb : (Lt >= 128 ? 128 : Lt >= 64 ? 64 : Lt > 16 ? 32 : 16);
     t.nsx = (Vs + cgu_tx(t.TB) - 1) / cgu_tx(t.TB);
     t.nB = t.nsx * ((Lt + t.TB - 1) / t.TB);
+    t.ns = band ? 1 : rhs_per_group(L, Lt, t.by, prec, nrhs, tu.ns);
     (void)units;
     return t;
 }
@@ -247,7 +262,20 @@ int launch_mdm_k(const MdmArgs<R>& a0, int nrhs, hipStream_t s) {
         set_band(a, c);
         blk = dim3((unsigned)(((c.nthr + 63) / 64) * 64));
     }
-    const dim3 grid((unsigned)(((a.Lt + a.T - 1) / a.T) * a.nband * nrhs));
+    // several right-hand sides per work-group (plain M^dag M from phi, whole slices)
+    int ns = 1;
+    if constexpr (MODE == MDM_PLAIN && PIPE && !HOPT) {
+        if (a.nsub > 1 && a.nband == 1 && a.prob == nullptr) {
+            const int nts = nthreads(a.L, BY);
+            const long long vbytes = (long long)a.vs * (long long)sizeof(cplx<R>);
+            ns = std::min(std::min(a.nsub, nrhs), (BY == 2 ? 1024 : 512) / nts);
+            while (ns > 1 && (ns * vbytes >= (1LL << 31) || ns * a.phi_cs * (long long)sizeof(R) >= (1LL << 31) ||
+                              (long long)a.L * a.L * a.Lt * 2 * (long long)sizeof(R) >= (1LL << 31))) --ns;
+            if (ns > 1) blk = dim3((unsigned)(((ns * nts + 63) / 64) * 64));
+        }
+    }
+    a.nsub = ns;
+    const dim3 grid((unsigned)(((a.Lt + a.T - 1) / a.T) * a.nband * ((nrhs + ns - 1) / ns)));
     const size_t lds = lds_bytes_c<R>(BY, (int)blk.x);
     if (blk.x <= 512) {
         auto k = mdagm_kernel<R, BY, MODE, 512, HOPT, PIPE>;
@@ -839,6 +867,22 @@ int qf_set_tuning_shape(int prec, int L, int Ltau, int nrhs, int by, int t, int 
     return QF_OK;
 }
 
+int qf_set_tuning_ns(int prec, int L, int Ltau, int nrhs, int ns) {
+    if (prec != QF_PREC_F32 && prec != QF_PREC_F64) return fail(QF_ERR_INVALID, "prec %d", prec);
+    if (ns < 0 || ns > 64 || nrhs < 0 || L < 2 || Ltau < 1) return fail(QF_ERR_INVALID, "bad shape or ns=%d", ns);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (auto& e : g_shape_tune)
+        if (e.prec == prec && e.L == L && e.Lt == Ltau && e.nrhs == nrhs) {
+            e.tu.ns = ns;
+            return QF_OK;
+        }
+    ShapeTuning e;
+    e.prec = prec; e.L = L; e.Lt = Ltau; e.nrhs = nrhs;
+    e.tu.ns = ns;
+    g_shape_tune.push_back(e);
+    return QF_OK;
+}
+
 int qf_clear_tuning(void) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_shape_tune.clear();
@@ -856,6 +900,14 @@ int qf_get_tiling(int prec, int nrhs, int L, int Ltau, int* out4) {
     if (rc) return rc;
     Tiling t = tiling(prec, nrhs, L, Ltau);
     out4[0] = t.by; out4[1] = t.T; out4[2] = t.TB; out4[3] = t.nA;
+    return QF_OK;
+}
+
+int qf_get_rhs_per_group(int prec, int nrhs, int L, int Ltau, int* ns) {
+    int rc = check_lattice(L, Ltau, nrhs);
+    if (rc) return rc;
+    if (!ns) return fail(QF_ERR_INVALID, "null pointer");
+    *ns = tiling(prec, nrhs, L, Ltau).ns;
     return QF_OK;
 }
 
@@ -940,7 +992,7 @@ int qf_apply(int op, int prec, const void* phi, long long phi_cs, const void* x,
         if (op == QF_OP_MDAGM) {
             MdmArgs<R> a{};
             a.phi = (const R*)phi; a.phi_cs = phi_cs; a.x = (const cplx<R>*)x; a.y = (cplx<R>*)y; a.vs = N;
-            a.L = L; a.Lt = Ltau; a.T = tl.T; a.hp = make_hop<R>(dtau, t);
+            a.L = L; a.Lt = Ltau; a.T = tl.T; a.hp = make_hop<R>(dtau, t); a.nsub = tl.ns;
             return launch_mdm<R, MDM_PLAIN>(a, tl.by, nrhs, s);
         }
         OpArgs<R> a{};

@@ -73,6 +73,10 @@ struct MdmArgs {
     // bands (nband > 1): each slice is split into nband bands of nown rows, each
     // work-group computing band rows plus hal halo rows on both sides (H rows in all)
     int nband, H, hal, nown;
+    // plain M^dag M from phi, whole slices: nsub > 1 runs the same tau run of nsub consecutive
+    // right-hand sides side by side in one work-group (lanes [k * nts, (k + 1) * nts) hold the
+    // slice of rhs rho + k), to fill whole waves where one slice leaves a wave mostly empty
+    int nsub;
 };
 
 // The band of work-group band index k (the whole lattice when nband <= 1): its own
@@ -139,6 +143,11 @@ __device__ __forceinline__ void mdagm_body(const MdmArgs<R>& A, int bid, int nbl
     int rho, run, rest, L, Lt, T, nBk;
     long long vo, pho, hoo, pa, pb;
     RowBand bd;
+    // several right-hand sides per work-group (plain M^dag M from phi only): the lane's slice
+    // is of rhs rho + sub; lanes of a group past the batch's end (sub >= nact) repeat rhs rho
+    // and store nothing
+    constexpr bool NSOK = MODE == MDM_PLAIN && PIPE && !HOPT;
+    int sub = 0, nact = 1, tloc = (int)threadIdx.x;
     if (A.prob != nullptr) {   // ragged batch: find this work-group's lattice
         int k = 0;
         for (int j = 1; j < A.nprob; ++j)
@@ -151,7 +160,15 @@ __device__ __forceinline__ void mdagm_body(const MdmArgs<R>& A, int bid, int nbl
     } else {
         // per rhs the work-groups are (run, band), band fastest
         const int nbd = A.nband > 1 ? A.nband : 1;
-        rho = bid % A.nrhs; rest = bid / A.nrhs; run = rest / nbd; L = A.L; Lt = A.Lt; T = A.T; nBk = A.nB;
+        const int ns = NSOK && A.nsub > 1 ? A.nsub : 1;
+        const int ngrp = (A.nrhs + ns - 1) / ns;
+        rho = (bid % ngrp) * ns; rest = bid / ngrp; run = rest / nbd; L = A.L; Lt = A.Lt; T = A.T; nBk = A.nB;
+        if (ns > 1) {
+            const int nts = (L >> 1) * (L / BY);   // lanes of one slice
+            sub = min(tloc / nts, ns - 1);
+            tloc -= sub * nts;
+            nact = min(ns, A.nrhs - rho);
+        }
         bd = band_of(L, A.nband, A.H, A.hal, A.nown, rest - run * nbd);
         vo = rho * A.vs; pho = rho * A.phi_cs; hoo = rho * A.hop_cs;
         pa = (long long)rho * (nblk / A.nrhs); pb = (long long)rho * A.nB;
@@ -159,11 +176,19 @@ __device__ __forceinline__ void mdagm_body(const MdmArgs<R>& A, int bid, int nbl
     const int Vs = L * L;
     const int t0 = run * T;
     const int t1 = min(t0 + T, Lt);
-    const Geo g = make_geo<BY>(L, threadIdx.x, bd);
-    const PatchOff po = patch_off<R>(g);
-    const XAddr xa = make_xaddr<BY>(g);
+    Geo g = make_geo<BY>(L, tloc, bd);
+    if (sub >= nact) g.own = false;
+    PatchOff po = patch_off<R>(g);
+    const XAddr xa = make_xaddr<BY>(g, (int)threadIdx.x - tloc);
     const unsigned vbytes = (unsigned)Vs * Lt * S;   // one vector
     const unsigned sstride = (unsigned)Vs * S;       // one slice of a vector
+    // the lane's rhs within the group: its vector and field offsets (the launcher keeps the
+    // group's vectors within 2^31 bytes); PatchOff poc addresses the angles of phi
+    PatchOff poc = po;
+    if (NSOK && sub > 0 && sub < nact) {
+        po.v += (unsigned)((long long)sub * A.vs * (long long)S);
+        poc.v += (unsigned)((long long)sub * A.phi_cs * (long long)sizeof(R));
+    }
     if constexpr (MODE == MDM_RESID) {
         if (A.ru_flag != nullptr && A.ru_flag[rho] == 0) return;   // uniform over the block
     }
@@ -171,7 +196,8 @@ __device__ __forceinline__ void mdagm_body(const MdmArgs<R>& A, int bid, int nbl
     const __amdgpu_buffer_rsrc_t rs_h = HOPT ? buf_rsrc(A.hop + hoo, 2u * vbytes) : buf_rsrc(A.x, 0u);
     // x-like inputs: MODE_CG reads z and p_old (x = z + beta p_old), else x
     const bool cg = MODE == MDM_CG;
-    const __amdgpu_buffer_rsrc_t rs_a = buf_rsrc(cg ? (const void*)(A.z + vo) : (const void*)(A.x + vo), vbytes);
+    const unsigned gbytes = NSOK ? (unsigned)((nact - 1) * A.vs * (long long)S) + vbytes : vbytes;   // the group's vectors
+    const __amdgpu_buffer_rsrc_t rs_a = buf_rsrc(cg ? (const void*)(A.z + vo) : (const void*)(A.x + vo), gbytes);
     const __amdgpu_buffer_rsrc_t rs_p = buf_rsrc(cg ? (const void*)(A.pold + vo) : (const void*)(A.x + vo), vbytes);
     const bool use_p = cg && A.iter > 0;
 
@@ -219,8 +245,9 @@ __device__ __forceinline__ void mdagm_body(const MdmArgs<R>& A, int bid, int nbl
     constexpr bool TOUCH = !PHIPF && !REGPF;
     // coefficient source of one slice: hop entries or phi
     const unsigned cslice = HOPT ? 2u * sstride : 2u * (unsigned)Vs * (unsigned)sizeof(R), nlanes = blockDim.x;
-    const __amdgpu_buffer_rsrc_t rs_c = HOPT ? rs_h : buf_rsrc(phi, cslice * (unsigned)Lt);
-    auto ldphi = [&](cplx<R> (&q)[2 * BY], int tau) { load_patch_buf<R, BY>(q, rs_c, po, (unsigned)tau * cslice, L); };
+    const unsigned cbytes = cslice * (unsigned)Lt + (NSOK ? (unsigned)((nact - 1) * A.phi_cs * (long long)sizeof(R)) : 0u);
+    const __amdgpu_buffer_rsrc_t rs_c = HOPT ? rs_h : buf_rsrc(phi, cbytes);
+    auto ldphi = [&](cplx<R> (&q)[2 * BY], int tau) { load_patch_buf<R, BY>(q, rs_c, poc, (unsigned)tau * cslice, L); };
     // Slices loaded ahead, clamped to the run: angles to its last slice, x to its halo slice
     // (loads past the end of a run then read a slice just read, an L2 hit, and are unused),
     // and wrapped cyclically.

@@ -516,18 +516,20 @@ struct XAddr {
     int nl;            // slots per plane
 };
 
+// base: first slot of the lane's slice (a work-group holding several slices side by side
+// gives each its own range of slots)
 template <int BY>
-__device__ __forceinline__ XAddr make_xaddr(const Geo& g) {
+__device__ __forceinline__ XAddr make_xaddr(const Geo& g, int base = 0) {
     const int na = g.L >> 1;
     XAddr x;
     x.me = threadIdx.x;
     x.nl = blockDim.x;
     const int am = g.a == 0 ? na - 1 : g.a - 1, ap = g.a + 1 == na ? 0 : g.a + 1;
     const int bm = g.b == 0 ? g.nb - 1 : g.b - 1, bp = g.b + 1 == g.nb ? 0 : g.b + 1;
-    x.l = g.b * na + am;
-    x.r = g.b * na + ap;
-    x.d = bm * na + g.a;
-    x.u = bp * na + g.a;
+    x.l = base + g.b * na + am;
+    x.r = base + g.b * na + ap;
+    x.d = base + bm * na + g.a;
+    x.u = base + bp * na + g.a;
     return x;
 }
 

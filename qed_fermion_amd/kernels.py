@@ -414,7 +414,15 @@ def clear_tuning():
 def get_tiling(prec, nrhs, L, Ltau):
     out = (ctypes.c_int * 4)()
     call("qf_get_tiling", int(prec), nrhs, L, Ltau, out)
-    return {"by": out[0], "t": out[1], "tb": out[2], "nA": out[3]}
+    ns = ctypes.c_int()
+    call("qf_get_rhs_per_group", int(prec), nrhs, L, Ltau, ctypes.byref(ns))
+    return {"by": out[0], "t": out[1], "tb": out[2], "nA": out[3], "ns": ns.value}
+
+
+def set_tuning_ns(prec, L, Ltau, nrhs=0, ns=0):
+    """Right-hand sides per work-group of the plain M^dag M from phi for one shape (0 or 1:
+    one); several fill whole wavefronts where one slice does not (see autotune.py)."""
+    call("qf_set_tuning_ns", int(prec), int(L), int(Ltau), int(nrhs), int(ns))
 
 
 # ------------------------------------------------------------------ ragged batch

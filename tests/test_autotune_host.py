@@ -34,3 +34,13 @@ def test_tau_run_candidates_fill_whole_rounds_of_work_groups():
     assert 45 in c                          # one per CU
     assert max(c) <= 48 and all(t <= 360 for t in c)
     assert autotune._t_candidates(10) == list(range(1, 11))
+
+
+def test_rhs_per_group_candidates():
+    assert autotune.ns_candidates(36, 2, 32) == [1, 2, 3]      # 324 lanes per slice, 1024 per work-group
+    assert autotune.ns_candidates(36, 4, 32) == [1, 2, 3]      # 162 lanes, 512 per work-group
+    assert autotune.ns_candidates(36, 6, 32) == [1, 2, 3, 4]   # 108 lanes; at most 4
+    assert autotune.ns_candidates(36, 2, 2) == [1, 2]          # not more than the batch
+    assert autotune.ns_candidates(36, 2, 1) == [1]
+    assert autotune.ns_candidates(64, 2, 8) == [1]             # a slice fills the work-group
+    assert autotune.ns_candidates(70, 2, 8) == [1]             # bands

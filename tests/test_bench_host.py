@@ -15,17 +15,17 @@ def test_traffic_follows_the_tiling():
         rec = json.load(f)
     seen = set()
     for e in rec["entries"]:
-        key = (e["kind"], e["nrhs"], e["tiling"]["by"], e["tiling"]["t"])
+        key = (e["kind"], e["nrhs"], e["tiling"]["by"], e["tiling"]["t"], e["tiling"].get("ns", 1))
         assert key not in seen                      # one entry per shape and tiling
         seen.add(key)
-        got = bench.traffic_of(e["L"], e["Ltau"], e["nrhs"], e["kind"], {"by": e["tiling"]["by"], "t": e["tiling"]["t"]})
+        got = bench.traffic_of(e["L"], e["Ltau"], e["nrhs"], e["kind"], dict(e["tiling"]))
         assert got == e["traffic_bytes"]
         # at least the algorithmic bytes; the excess is the tau halo, which shrinks with the run length
         assert e["algorithmic_bytes"] <= e["traffic_bytes"] < 1.25 * e["algorithmic_bytes"]
     for kind in ("per_chain", "per_chain_phi"):
         for by in (2, 4):
             runs = sorted((e["tiling"]["t"], e["traffic_bytes"]) for e in rec["entries"]
-                          if e["kind"] == kind and e["tiling"]["by"] == by)
+                          if e["kind"] == kind and e["tiling"]["by"] == by and e["tiling"].get("ns", 1) == 1)
             assert [b for _, b in runs] == sorted((b for _, b in runs), reverse=True)   # longer runs, less halo
     # from phi: (24 + 8 * 3 / t) bytes per site per chain, the tau halo of a t-slice run
     for e in rec["entries"]:
@@ -33,6 +33,8 @@ def test_traffic_follows_the_tiling():
             t = e["tiling"]["t"]
             assert abs(e["traffic_over_algorithmic"] - (24 + 24 / t) / 24) < 0.01, e["tiling"]
     assert bench.traffic_of(36, 360, 32, "per_chain_phi", {"by": 2, "t": 24}) is not None
+    assert bench.traffic_of(36, 360, 32, "per_chain_phi", {"by": 2, "t": 24, "ns": 1}) is not None
+    assert bench.traffic_of(36, 360, 32, "per_chain_phi", {"by": 2, "t": 7, "ns": 2}) is None   # not measured
     assert bench.traffic_of(36, 360, 32, "per_chain", {"by": 6, "t": 7}) is None      # not measured
     assert bench.traffic_of(36, 360, 31, "per_chain", {"by": 4, "t": 12}) is None     # other shape
 

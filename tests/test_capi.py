@@ -74,6 +74,22 @@ def test_argument_validation_without_gpu():
     out = (ctypes.c_int * 4)()
     assert h.qf_get_tiling(0, 1, 36, 360, out) == 0 and out[0] in (2, 4, 6)
     assert h.qf_get_tiling(0, 1, 66, 660, out) == 0 and out[0] == 6
+    # right-hand sides per work-group of the plain M^dag M from phi: capped by the work-group
+    # and the batch, one on band lattices; a shape entry keeps its tiling
+    ns = ctypes.c_int()
+    try:
+        assert h.qf_set_tuning_shape(0, 36, 360, 32, 2, 23, 0) == 0
+        assert h.qf_set_tuning_ns(0, 36, 360, 32, 7) == 0
+        assert h.qf_get_rhs_per_group(0, 32, 36, 360, ctypes.byref(ns)) == 0 and ns.value == 3   # 3 x 324 lanes
+        assert h.qf_get_tiling(0, 32, 36, 360, out) == 0 and (out[0], out[1]) == (2, 23)
+        assert h.qf_set_tuning_ns(0, 36, 360, 2, 3) == 0
+        assert h.qf_get_rhs_per_group(0, 2, 36, 360, ctypes.byref(ns)) == 0 and ns.value == 2    # the batch
+        assert h.qf_set_tuning_ns(0, 70, 700, 8, 3) == 0
+        assert h.qf_get_rhs_per_group(0, 8, 70, 700, ctypes.byref(ns)) == 0 and ns.value == 1    # bands
+        assert h.qf_get_rhs_per_group(0, 32, 36, 100, ctypes.byref(ns)) == 0 and ns.value == 1   # untuned shape
+        assert h.qf_set_tuning_ns(0, 36, 360, 32, -1) == _lib.QF_ERR_INVALID
+    finally:
+        h.qf_clear_tuning()
     # the allocation-free solver refuses a solve larger than its static scratch before any launch
     assert h.qf_pcg_noalloc_max_bytes() == 64 << 20
     rc = h.qf_pcg_solve_noalloc(1, None, 0, None, None, 64, 36, 360, 0.1, 1.0, None, 1e-9, 10, None, None, None, None)

@@ -306,6 +306,38 @@ def test_operator_repeat_multiwave(dev, prec, tol, L, by):
         k.set_tuning(pi, 0, 0, 0)
 
 
+@pytest.mark.parametrize("L,Lt,nrhs,shared,by", [(36, 12, 5, False, 2), (36, 12, 7, True, 2), (12, 20, 9, False, 2),
+                                                  (24, 10, 4, False, 4), (6, 12, 11, False, 2), (32, 8, 5, False, 2)])
+def test_rhs_per_group(dev, L, Lt, nrhs, shared, by):
+    """Plain M^dag M from phi with several right-hand sides side by side in one work-group:
+    bit-identical to one per work-group for every group size (the lanes of each slice compute
+    the same patches; the last group of a batch that ns does not divide leaves lanes idle), in
+    both precisions, per-chain and shared fields; FP64 within 1e-12 of the oracle."""
+    import hashlib
+    k = _k()
+    print(f"[rhs per group] L={L} Lt={Lt} nrhs={nrhs} shared={shared} by={by}", flush=True)
+    nch = 1 if shared else nrhs
+    phi = make_phi(L, Lt, "uniform", seed=L + nrhs, nchain=nch)
+    x = make_vec(L, Lt, seed=L + 2 * nrhs, nrhs=nrhs)
+    try:
+        for pi, rd, cd in ((0, torch.float32, torch.complex64), (1, torch.float64, torch.complex128)):
+            pd, xd = _t(phi.reshape(nch, -1), dev, rd), _t(x, dev, cd)
+            k.set_tuning_shape(pi, L, Lt, nrhs, by, 3, 0)
+            seen = {}
+            for ns in (1, 2, 3, 4, 7):
+                k.set_tuning_ns(pi, L, Lt, nrhs, ns)
+                eff = k.get_tiling(pi, nrhs, L, Lt)["ns"]
+                y = k.apply("MdagM", pd, xd, L, Lt).cpu().numpy()
+                seen[(ns, eff)] = hashlib.sha1(y.tobytes()).hexdigest()
+                if pi == 1 and ns == 3:
+                    for i in range(nrhs):
+                        assert relerr(y[i], O.apply_MdagM(phi[0 if shared else i], x[i], L, Lt)) < 1e-12, i
+            assert len(set(seen.values())) == 1, (pi, seen)
+            assert max(e for _, e in seen) > 1
+    finally:
+        k.clear_tuning()
+
+
 @pytest.mark.parametrize("T", [12, 15, 16, 24, 30])
 def test_long_tau_runs(dev, T):
     """tau runs of 12 to 30 slices, the autotuner's longest candidates, on Ntau = 50 (so that

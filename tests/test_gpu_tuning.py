@@ -25,13 +25,22 @@ def test_tune_operator_and_pcg(dev, tmp_path, monkeypatch):
         y = kernels.apply_hop("MdagM", hop, torch.from_numpy(x).to("cuda", torch.complex64), L, Lt).cpu().numpy()
         for i in range(nrhs):
             assert relerr(y[i], O.apply_MdagM(phi[0], x[i], L, Lt)) < 2e-5
+        print("[autotune] operator from phi, per-chain fields", flush=True)
+        hit3 = autotune.tune(0, L, Lt, nrhs, kind="operator", shared_field=False, source="phi", reps=3)
+        assert hit3["ns"] in autotune.ns_candidates(L, hit3["by"], nrhs)
+        assert kernels.get_tiling(0, nrhs, L, Lt)["ns"] == hit3["ns"]
+        php = make_phi(L, Lt, "uniform", seed=5, nchain=nrhs)
+        y = kernels.apply("MdagM", torch.from_numpy(php.reshape(nrhs, -1)).to("cuda", torch.float32),
+                          torch.from_numpy(x).to("cuda", torch.complex64), L, Lt).cpu().numpy()
+        for i in range(nrhs):
+            assert relerr(y[i], O.apply_MdagM(php[i], x[i], L, Lt)) < 2e-5
         print("[autotune] pcg", flush=True)
         hit2 = autotune.tune(1, L, Lt, 2, kind="pcg", iters=5, reps=1)
         assert kernels.get_tiling(1, 2, L, Lt)["tb"] == hit2["tb"] or hit2["tb"] == 0
         # cached: a second call reuses the entry
         assert autotune.tune(1, L, Lt, 2, kind="pcg") == hit2
         kernels.clear_tuning()
-        assert autotune.load() == 2
+        assert autotune.load() == 3
         phd = torch.from_numpy(make_phi(L, Lt, "gauss", seed=3, nchain=2).reshape(2, -1)).to("cuda", torch.float64)
         R = torch.from_numpy(make_vec(L, Lt, seed=4, nrhs=2)).to("cuda", torch.complex128)
         b = kernels.apply("Mdag", phd, R, L, Lt)
