@@ -265,7 +265,7 @@ int launch_mdm_k(const MdmArgs<R>& a0, int nrhs, hipStream_t s) {
     // several right-hand sides per work-group (plain M^dag M from phi, whole slices)
     int ns = 1;
     if constexpr (MODE == MDM_PLAIN && PIPE && !HOPT) {
-        if (a.nsub > 1 && a.nband == 1 && a.prob == nullptr) {
+        if (a.nsub > 1 && a.nband == 1 && a.prob == nullptr && a.dot_part == nullptr) {   // partials are per (rhs, run)
             const int nts = nthreads(a.L, BY);
             const long long vbytes = (long long)a.vs * (long long)sizeof(cplx<R>);
             ns = std::min(std::min(a.nsub, nrhs), (BY == 2 ? 1024 : 512) / nts);
