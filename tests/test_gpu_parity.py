@@ -338,6 +338,35 @@ def test_rhs_per_group(dev, L, Lt, nrhs, shared, by):
         k.clear_tuning()
 
 
+def test_rhs_per_group_ignored_where_not_supported(dev):
+    """A request for several right-hand sides per work-group leaves the forms without it
+    unchanged: band lattices (L=70) and the M^dag M from the hop table give the same bits as
+    without the request, and the band result matches the oracle."""
+    import hashlib
+    k = _k()
+    print("[rhs per group ignored]", flush=True)
+    try:
+        for L, Lt, nrhs in ((70, 4, 3), (12, 8, 4)):
+            phi = make_phi(L, Lt, "uniform", seed=L + 11, nchain=nrhs)
+            x = make_vec(L, Lt, seed=L + 12, nrhs=nrhs)
+            pd, xd = _t(phi.reshape(nrhs, -1), dev, torch.float32), _t(x, dev, torch.complex64)
+            hop = k.hop_table(pd, L, Lt)
+            out = {}
+            for ns in (0, 3):
+                k.set_tuning_ns(0, L, Lt, nrhs, ns)
+                ys = [k.apply_hop("MdagM", hop, xd, L, Lt).cpu().numpy()]
+                if L == 70:
+                    ys.append(k.apply("MdagM", pd, xd, L, Lt).cpu().numpy())
+                out[ns] = [hashlib.sha1(y.tobytes()).hexdigest() for y in ys]
+            assert out[0] == out[3], (L, out)
+            if L == 70:
+                assert k.get_tiling(0, nrhs, L, Lt)["ns"] == 1
+                for i in range(nrhs):
+                    assert relerr(ys[1][i], O.apply_MdagM(phi[i], x[i], L, Lt)) < 2e-5
+    finally:
+        k.clear_tuning()
+
+
 @pytest.mark.parametrize("T", [12, 15, 16, 24, 30])
 def test_long_tau_runs(dev, T):
     """tau runs of 12 to 30 slices, the autotuner's longest candidates, on Ntau = 50 (so that
