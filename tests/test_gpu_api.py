@@ -180,6 +180,31 @@ def test_tau_block_updates(dev, cuda_graph):
     assert small.max_tau_block_idx == 1 and small.tau_block_size == 30
 
 
+def test_z2_noise_is_orthonormal_on_average(dev):
+    """random_vec_bin draws on the device (stochastic_estimator.py:210-224): every entry is one
+    of (+-1 +- i)/sqrt(2), so |eta_i|^2 = 1 exactly, and the noise average of eta eta^dag is the
+    identity up to O(1/sqrt(Nrv)) off the diagonal (the reference's orthogonality check,
+    unit_test/test_stochastic_estimator.py:52-54), with the four phases equally likely."""
+    from qed_fermion_amd.stochastic_estimator import StochaticEstimator
+    L, Lt, nrv = 2, 30, 8192
+    torch.manual_seed(5)
+    hmc = _sampler(L, Lt, 1, "fp64")
+    se = StochaticEstimator(hmc, cuda_graph_se=False, Nrv=nrv)
+    e = se.random_vec_bin().to(torch.complex128).cpu().numpy()     # [Nrv, N]
+    N = e.shape[1]
+    assert np.allclose(np.abs(e) ** 2, 1.0, atol=1e-12)
+    C = e.T @ e.conj() / nrv                                         # (1/Nrv) sum_s eta_s eta_s^dag
+    assert np.allclose(np.diag(C), 1.0, atol=1e-12)
+    off = np.abs(C - np.diag(np.diag(C)))
+    assert off.max() < 6.0 / math.sqrt(nrv), off.max()
+    assert abs(np.abs(C).sum() - N) / (N * (N - 1)) < 2.0 / math.sqrt(nrv)   # no systematic correlation
+    ph = np.round(np.angle(e) / (np.pi / 4)).astype(int) % 8         # 1, 3, 5, 7: the four phases
+    counts = np.bincount(ph.ravel(), minlength=8)
+    assert counts[[0, 2, 4, 6]].sum() == 0
+    frac = counts[[1, 3, 5, 7]] / ph.size
+    assert np.all(np.abs(frac - 0.25) < 6 * math.sqrt(0.25 * 0.75 / ph.size)), frac
+
+
 def test_stochastic_estimator_solve(dev):
     from qed_fermion_amd.stochastic_estimator import StochaticEstimator
     L, Lt = 4, 30
