@@ -175,10 +175,12 @@ Tiling tiling(int prec, int nrhs, int L, int Lt) {
     const bool band = !square_ok(L) || band_mode(L, tu.by);
     t.by = band ? 2 : pick_by(L, prec, tu.by);
     const long long units = (long long)nrhs * Lt;
-    // tau slices per operator work-group: about 2048 work-groups in flight
-    // over the batch, at least 4 slices so the per-run prologue (one extra B
-    // application) stays small, at most 16
-    t.T = tu.t ? tu.t : (int)std::max(4LL, std::min(16LL, units / 2048));
+    // tau slices per operator work-group (untuned): about 512 work-groups over the batch, two
+    // per CU, which the autotuner's picks follow (8 chains at L=36, Ltau=360: 6 slices; 32
+    // chains: 23), at least 4 slices so the per-run prologue (one extra B application) stays
+    // small, at most 24
+    const long long nbands = band ? band_cfg(L, kHalMdm).nband : 1;
+    t.T = tu.t ? tu.t : (int)std::max(4LL, std::min(24LL, (units * nbands + 511) / 512));
     t.T = std::min(t.T, Lt);
     t.nA = (Lt + t.T - 1) / t.T;
     if (band) t.nA *= band_cfg(L, kHalMdm).nband;   // one partial sum per (run, band)
