@@ -493,6 +493,7 @@ def extras(torch, K, args, dev):
         Nc = Lc * Lc * Ltc
         phc = rng.uniform(-np.pi, np.pi, 2 * Nc) if kind == "uniform" else 0.3 * rng.standard_normal(2 * Nc)
         phc = torch.from_numpy(phc).view(1, -1).to(dev)
+        tune_c = autotune.tune(1, Lc, Ltc, 1, kind="pcg", persist=False)
         eng = TrajectoryEngine(Lc, Ltc, 1, precision="fp64", dtau=0.1, J=1.0, K=1.0, compact=False, m=1.0, delta_t=dtc,
                                n_leap=nl, max_iter=4000, rtol=1e-10, device=dev)
         eng.set_phi(phc)
@@ -509,7 +510,7 @@ def extras(torch, K, args, dev):
             eng.set_phi(phc)
             eng.replay() if graph else eng.trajectory()
         tc = timed(torch, one, reps=3)
-        out[name] = {"s_per_trajectory": tc, "solves": nl + 1, "dt": dtc, "field": kind, "graph": graph,
+        out[name] = {"s_per_trajectory": tc, "solves": nl + 1, "dt": dtc, "field": kind, "graph": graph, "tiling": tune_c,
                      "iterations_per_solve_eager": [min(need), max(need)], "caps_total": sum(eng.max_iters),
                      "max_residual": eng.err.max().item(), "solves_at_cap": int((eng.status == 1).sum().item())}
         log(f"[bench] {name}: {tc * 1e3:.1f} ms per trajectory")

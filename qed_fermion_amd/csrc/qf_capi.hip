@@ -175,12 +175,15 @@ Tiling tiling(int prec, int nrhs, int L, int Lt) {
     const bool band = !square_ok(L) || band_mode(L, tu.by);
     t.by = band ? 2 : pick_by(L, prec, tu.by);
     const long long units = (long long)nrhs * Lt;
-    // tau slices per operator work-group (untuned): about 512 work-groups over the batch, two
-    // per CU, which the autotuner's picks follow (8 chains at L=36, Ltau=360: 6 slices; 32
-    // chains: 23), at least 4 slices so the per-run prologue (one extra B application) stays
-    // small, at most 24
+    // tau slices per operator work-group (untuned), where the autotuner lands: about 512
+    // work-groups over the batch (two per CU; 8 chains at L=36, Ltau=360: 6 slices, 32 chains:
+    // 23), 256 for bands (one work-group of 9 or more waves per CU; one chain at L=66: 6), at
+    // most 24 slices.  Small batches get runs down to one slice: their launches are
+    // latency-bound, and the extra B application of each run's prologue costs less than
+    // marching (one FP64 chain at L=24, Ltau=240: 1 slice, 35% faster per trajectory than 4).
     const long long nbands = band ? band_cfg(L, kHalMdm).nband : 1;
-    t.T = tu.t ? tu.t : (int)std::max(4LL, std::min(24LL, (units * nbands + 511) / 512));
+    const long long target = band ? 256 : 512;
+    t.T = tu.t ? tu.t : (int)std::max(1LL, std::min(24LL, (units * nbands + target - 1) / target));
     t.T = std::min(t.T, Lt);
     t.nA = (Lt + t.T - 1) / t.T;
     if (band) t.nA *= band_cfg(L, kHalMdm).nband;   // one partial sum per (run, band)
